@@ -1,0 +1,179 @@
+/*
+ * pfdtd.h -- C ABI of the MI355X (gfx950) SRL_FORWARD time-stepping core.
+ *
+ * This is the boundary between host code (the FDTD::App facade, Python through ctypes or
+ * pybind11, a MEX gateway) and the hand-written HIP kernels in libpfdtd.so. Plain C types
+ * only: no HIP or torch types cross it. Every entry point returns an int status
+ * (PFDTD_OK == 0, negative on error) and leaves a message for pfdtd_last_error().
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * juuli/ParallelFDTD repository root).
+ *
+ * Grid layout everywhere: idx = (z*Y + y)*X + x, x fastest (src/kernels/cudaMesh.h:247-249).
+ * Precondition (checked): every voxel of the outer layer is outside (position byte bit 7
+ * clear); the reference only updates slices z in [1, Z-2] and wraps x/y edges into
+ * neighbouring rows (src/kernels/kernels3d.cu:112-123, :516-521), which is only well
+ * defined under this precondition.
+ */
+#ifndef PFDTD_H
+#define PFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------ */
+#define PFDTD_OK 0
+#define PFDTD_E_ARG (-1)    /* bad argument (null pointer, size, index, enum value) */
+#define PFDTD_E_HIP (-2)    /* a HIP runtime call failed */
+#define PFDTD_E_NOMEM (-3)  /* device or host allocation failed / request exceeds free HBM */
+#define PFDTD_E_STATE (-4)  /* call order: e.g. run before a domain and materials are set */
+#define PFDTD_E_DOMAIN (-5) /* domain violates a precondition (shell, K range, material slot) */
+
+/* ---- field precision --------------------------------------------------------------- */
+#define PFDTD_F32 0  /* FP32 fields and arithmetic: launchFDTD3d */
+#define PFDTD_F64 1  /* FP64 fields and arithmetic: launchFDTD3dDouble */
+#define PFDTD_F16 2  /* FP32 arithmetic, binary16 field storage, power-of-two scaled */
+#define PFDTD_BF16 3 /* FP32 arithmetic, bfloat16 field storage, power-of-two scaled */
+
+/* ---- source kinds: enum SrcType, src/base/SrcRec.h:33 ----------------------------- */
+#define PFDTD_SRC_HARD 0
+#define PFDTD_SRC_SOFT 1
+#define PFDTD_SRC_TRANSPARENT 2 /* injected like SOFT; the signal is pre-filtered by the host */
+
+/* ---- material-slot and soft-source conventions (reference quirks) ------------------ */
+#define PFDTD_MATSLOT_INTENDED 0 /* beta = table[mat*20 + octave] (kernels3d.cu:639) */
+#define PFDTD_MATSLOT_LITERAL 1  /* beta = table[mat*20*octave]   (kernels3d.cu:513) */
+#define PFDTD_SOFT_ADD 0         /* soft source adds (CudaMeshTest.cpp:311-314) */
+#define PFDTD_SOFT_OVERWRITE 2   /* soft source overwrites, as addSample does (cudaMesh.h:362-366) */
+
+typedef struct pfdtd_ctx pfdtd_ctx;
+
+/* InterruptCallback bool(*)(void), src/App.h:33 -- nonzero stops the run. */
+typedef int (*pfdtd_interrupt_cb)(void *user);
+/* ProgressCallback void(*)(int,int,float), src/App.h:34 -- called every 100 steps. */
+typedef void (*pfdtd_progress_cb)(int step, int n_steps, float seconds_per_step, void *user);
+
+/* Last error message of the calling thread (cudasafe's message, src/kernels/cudaUtils.h:47-51). */
+const char *pfdtd_last_error(void);
+int pfdtd_version(void);
+
+/* Device queries: App::queryDevices, src/App.cpp:91-111. */
+int pfdtd_device_count(int *count);
+int pfdtd_device_mem(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* Context on one device: App::initializeDevices, src/App.cpp:125-131 (without resetting the
+ * device: other contexts may share it). All device memory belongs to the context. */
+int pfdtd_ctx_create(int device, pfdtd_ctx **out);
+/* App::close / CudaMesh::destroyPartitions, src/App.cpp:438-443, src/kernels/cudaMesh.h:160-182. */
+int pfdtd_ctx_destroy(pfdtd_ctx *ctx);
+
+/* ---- domain ------------------------------------------------------------------------- */
+/* Pre-classified position and material bytes: CudaMesh::setupMesh(+Double),
+ * src/kernels/cudaMesh.cu:26-153 (without block padding). Position byte = 0x80|K for inside
+ * voxels (K = air face-neighbour count, 0..6), 0 outside. Copies the host arrays. */
+int pfdtd_set_domain(pfdtd_ctx *ctx, const uint8_t *pos, const uint8_t *mat, int64_t X,
+                     int64_t Y, int64_t Z);
+/* Label grid classified on the GPU: label 0 = inside, label m+1 = solid voxel of material m.
+ * Replaces voxelizeGeometry + toBilbaoScheme (src/kernels/voxelizationUtils.cu:55-136,
+ * src/kernels/cudaMesh.cu:204-251) for voxel-built rooms: K = inside face neighbours
+ * (out-of-grid counts as solid), material byte = material of the first solid face neighbour
+ * in the order -x,+x,-y,+y,-z,+z for inside voxels with K<6, else 0. */
+int pfdtd_set_domain_labels(pfdtd_ctx *ctx, const uint8_t *labels, int64_t X, int64_t Y,
+                            int64_t Z);
+/* Voxelizer node codes (0..27) mapped on the GPU by toBilbao, src/kernels/cudaMesh.cu:328-361
+ * (toBilbaoKernel :492-498); material bytes as given (zeroed for code 0). */
+int pfdtd_set_domain_codes(pfdtd_ctx *ctx, const uint8_t *codes, const uint8_t *mat, int64_t X,
+                           int64_t Y, int64_t Z);
+/* Device -> host copy of the classified bytes (bit-exactness checks). */
+int pfdtd_get_domain(pfdtd_ctx *ctx, uint8_t *pos, uint8_t *mat);
+int pfdtd_get_dims(pfdtd_ctx *ctx, int64_t *X, int64_t *Y, int64_t *Z);
+/* CudaMesh::getNumberOfAirElements / getNumberOfBoundaryElements (src/kernels/cudaMesh.h:242-243),
+ * counted like calcBoundaries, src/kernels/cudaMesh.cu:500-514. */
+int pfdtd_get_counts(pfdtd_ctx *ctx, uint64_t *air, uint64_t *boundary);
+
+/* ---- materials and parameters ------------------------------------------------------ */
+/* Material table [n_mat][n_coef] of admittances (MaterialHandler::getMaterialCoefficientPtr,
+ * src/base/MaterialHandler.cpp:100-164; coefficients beyond n_coef up to 20 are zero) and the
+ * parameter vector [lambda, lambda^2, 1/3, octave] (SimulationParameters::getParameterPtr,
+ * src/base/SimulationParameters.cpp:379-396). flags: PFDTD_MATSLOT_* | PFDTD_SOFT_*.
+ * The per-(K, material) update coefficients are built from these on the host in the
+ * reference's own operation order (FP32 and FP64 tables). */
+int pfdtd_set_materials(pfdtd_ctx *ctx, const float *table, int n_mat, int n_coef,
+                        unsigned octave, double lambda, int flags);
+
+/* ---- sources and receivers ----------------------------------------------------------- */
+/* Sources at grid indices xyz[3*n] (already +1 padded as
+ * SimulationParameters::getSourceElementCoordinates, SimulationParameters.cpp:200-211),
+ * kinds PFDTD_SRC_*, and their signals [n_src][n_samples] as the FP32 and FP64 runs see them
+ * (getSourceSample / getSourceSampleDouble, SimulationParameters.cpp:139-159). Either signal
+ * array may be NULL, in which case it is converted from the other. Steps past n_samples
+ * inject 0. Sources outside the grid are skipped (setSample, cudaMesh.h:321-338). */
+int pfdtd_set_sources(pfdtd_ctx *ctx, int n_src, const int32_t *xyz, const int32_t *kind,
+                      int n_samples, const float *signal_f32, const double *signal_f64);
+/* Receivers at grid indices xyz[3*n] (getReceiverElementCoordinates,
+ * SimulationParameters.cpp:213-224). Receivers outside the grid record 0 (kernels3d.cu:168). */
+int pfdtd_set_receivers(pfdtd_ctx *ctx, int n_rec, const int32_t *xyz);
+
+/* ---- running ------------------------------------------------------------------------ */
+/* Allocate (if needed) and zero the two pressure fields in the given precision.
+ * scale_exp: fields are stored multiplied by 2^scale_exp (F16/BF16 only; 0 otherwise).
+ * CudaMesh::makePartition's field allocation (cudaMesh.h:700-746) + resetPressures
+ * (cudaMesh.h:782-791, App::resetPressureMesh src/App.cpp:397-401). */
+int pfdtd_prepare_fields(pfdtd_ctx *ctx, int precision, int scale_exp);
+int pfdtd_reset_fields(pfdtd_ctx *ctx);
+
+/* Whole run from step 0: launchFDTD3d / launchFDTD3dDouble, src/kernels/kernels3d.cu:31-374.
+ * Fields are zeroed first. Responses stay on the device until pfdtd_get_responses_*.
+ * interrupt/progress may be NULL; they are polled between step bundles (every 100 steps for
+ * progress). steps_done receives the number of completed steps (< n_steps when interrupted;
+ * later samples are 0, kernels3d.cu:86-89). ms_elapsed (optional) receives device time of the
+ * whole step loop measured with HIP events on the context's stream. */
+int pfdtd_run(pfdtd_ctx *ctx, int precision, int n_steps, pfdtd_interrupt_cb interrupt,
+              pfdtd_progress_cb progress, void *user, int *steps_done, double *ms_elapsed);
+
+/* One step at time index `step` (source samples of that step, receivers written to response
+ * sample `step`): launchFDTD3dStep, src/kernels/kernels3d.cu:376-482, used by App::executeStep
+ * (src/App.cpp:407-436). direction is +1 or -1; when it differs from the previous call's the
+ * field pointers are not flipped (time reversal, kernels3d.cu:462-465). Requires
+ * pfdtd_prepare_fields and a response length (n_steps) set by pfdtd_begin_steps. */
+int pfdtd_begin_steps(pfdtd_ctx *ctx, int precision, int n_steps);
+int pfdtd_step(pfdtd_ctx *ctx, int step, int direction);
+
+/* Responses [n_rec][n_steps] of the last run or step sequence (kernels3d.cu:184-191). */
+int pfdtd_get_responses_f32(pfdtd_ctx *ctx, float *out, int64_t capacity);
+int pfdtd_get_responses_f64(pfdtd_ctx *ctx, double *out, int64_t capacity);
+int pfdtd_get_response_shape(pfdtd_ctx *ctx, int *n_rec, int *n_steps);
+
+/* Slice of the current pressure field and of the position bytes: captureSliceFast,
+ * src/kernels/visualizationUtils.cu:130-254 (captureSliceKernel :500-527). orientation 0 = xy
+ * (out[y*X+x] at z=slice), 1 = xz (out[z*X+x] at y=slice), 2 = yz (out[z*Y+y] at x=slice).
+ * Values are unscaled FP32. PFDTD_E_ARG when the slice is out of range (the reference logs and
+ * skips the capture). */
+int pfdtd_capture_slice(pfdtd_ctx *ctx, int64_t slice, int orientation, float *p_out,
+                        uint8_t *pos_out);
+/* Whole current field as unscaled FP32: captureMesh, visualizationUtils.cu:111-128. */
+int pfdtd_capture_field(pfdtd_ctx *ctx, float *out, int64_t capacity);
+/* Whole current and past fields in storage precision (raw bytes), for checks. */
+int pfdtd_get_fields_raw(pfdtd_ctx *ctx, void *p_out, void *p_past_out, int64_t capacity_bytes);
+
+/* Timing of the update kernel inside pfdtd_run: when enabled, every update launch is bracketed
+ * by HIP events on the context's stream and the summed device time is reported. */
+int pfdtd_set_kernel_timing(pfdtd_ctx *ctx, int enabled);
+/* Steps between host polls of the interrupt/progress callbacks inside pfdtd_run (default 100,
+ * PROGRESS_MOD in src/kernels/kernels3d.h:36). The reference polls the interrupt every step
+ * (kernels3d.cu:86); polling here costs one stream synchronisation per interval. */
+int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
+/* Update-kernel tiling: y rows per thread (1, 2 or 4; default 2) and slices per z-chunk
+ * (default 8). Results do not depend on it (every voxel's arithmetic is the same); it only
+ * changes the streaming pattern. Replaces the fixed 32x4 block of App::initializeMesh
+ * (src/App.cpp:193) and the one-block-per-slice grid of launchFDTD3d (kernels3d.cu:42-45). */
+int pfdtd_set_update_tiling(pfdtd_ctx *ctx, int rows, int zchunk);
+int pfdtd_get_kernel_timing(pfdtd_ctx *ctx, double *update_ms_total, int64_t *update_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

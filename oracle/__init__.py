@@ -1,0 +1,134 @@
+"""CPU parity oracle (test infrastructure).
+
+ctypes binding of ``oracle/liboracle.so`` (built from ``pfdtd_oracle.c`` by ``oracle/Makefile``).
+Only the tests, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg import this
+package, as the checker; the product library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+_lib = None
+
+_p = C.c_void_p
+_i64 = C.c_int64
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_to_bilbao.restype = C.c_uint8
+        L.orc_to_bilbao.argtypes = [C.c_uint8, _p]
+        L.orc_classify_labels.restype = None
+        L.orc_classify_labels.argtypes = [_p, _i64, _i64, _i64, _p, _p, _p]
+        L.orc_param_vector_f.restype = None
+        L.orc_param_vector_f.argtypes = [C.c_double, C.c_uint, _p]
+        L.orc_param_vector_d.restype = None
+        L.orc_param_vector_d.argtypes = [C.c_double, C.c_uint, _p]
+        L.orc_r2y_f.restype = C.c_float
+        L.orc_r2y_f.argtypes = [C.c_float]
+        L.orc_dx.restype = C.c_float
+        L.orc_dx.argtypes = [C.c_float, C.c_uint, C.c_double]
+        L.orc_element_idx.restype = None
+        L.orc_element_idx.argtypes = [C.c_float, C.c_float, C.c_float, C.c_uint, C.c_float,
+                                      C.c_double, C.c_int, _p]
+        L.orc_source_sample_f.restype = C.c_float
+        L.orc_source_sample_f.argtypes = [C.c_int, C.c_uint, C.c_uint, _p, C.c_uint]
+        L.orc_source_sample_d.restype = C.c_double
+        L.orc_source_sample_d.argtypes = [C.c_int, C.c_uint, C.c_uint, _p, C.c_uint]
+        L.orc_transparent_sample_f.restype = C.c_float
+        L.orc_transparent_sample_f.argtypes = [C.c_int, C.c_uint, C.c_uint, _p, C.c_uint, _p, C.c_uint]
+        L.orc_transparent_sample_d.restype = C.c_double
+        L.orc_transparent_sample_d.argtypes = [C.c_int, C.c_uint, C.c_uint, _p, C.c_uint, _p, C.c_uint]
+        L.orc_material_slot.restype = C.c_int64
+        L.orc_material_slot.argtypes = [C.c_uint, C.c_uint, C.c_int]
+        L.orc_run_f.restype = C.c_int
+        L.orc_run_f.argtypes = [_p, _p, _i64, _i64, _i64, _p, C.c_int, C.c_uint, C.c_double,
+                                _p, _p, _p, C.c_int, _p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_int, _p, _p, _p]
+        L.orc_run_d.restype = C.c_int
+        L.orc_run_d.argtypes = [_p, _p, _i64, _i64, _i64, _p, C.c_int, C.c_uint, C.c_double,
+                                _p, _p, _p, C.c_int, _p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                _p, _p, _p]
+        L.orc_capture_slice_f.restype = C.c_int
+        L.orc_capture_slice_f.argtypes = [_p, _p, _i64, _i64, _i64, _i64, C.c_int, _p, _p]
+        L.orc_f32_to_f16.restype = C.c_uint16
+        L.orc_f32_to_f16.argtypes = [C.c_float]
+        L.orc_f16_to_f32.restype = C.c_float
+        L.orc_f16_to_f32.argtypes = [C.c_uint16]
+        L.orc_f32_to_bf16.restype = C.c_uint16
+        L.orc_f32_to_bf16.argtypes = [C.c_float]
+        L.orc_bf16_to_f32.restype = C.c_float
+        L.orc_bf16_to_f32.argtypes = [C.c_uint16]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def classify(labels: np.ndarray):
+    """(pos, mat, counts[9]) for a (Z, Y, X) label grid."""
+    labels = np.ascontiguousarray(labels, np.uint8)
+    Z, Y, X = labels.shape
+    pos = np.empty_like(labels)
+    mat = np.empty_like(labels)
+    counts = np.zeros(9, np.uint64)
+    lib().orc_classify_labels(_ptr(labels), X, Y, Z, _ptr(pos), _ptr(mat), _ptr(counts))
+    return pos, mat, counts
+
+
+STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+
+
+def run(pos, mat, table, src_xyz, src_kind, signals, rec_xyz, n_steps, *, precision="f32",
+        octave=0, lam=math.sqrt(1.0 / 3.0), literal=0, storage=STORE_F32, scale_exp=0,
+        threads=1, want_fields=False):
+    """Reference step loop on the CPU. Returns responses [n_rec][n_steps] (and the final
+    current/past fields when want_fields)."""
+    pos = np.ascontiguousarray(pos, np.uint8)
+    mat = np.ascontiguousarray(mat, np.uint8)
+    Z, Y, X = pos.shape
+    dt = np.float64 if precision == "f64" else np.float32
+    t = np.ascontiguousarray(table, np.float32)
+    if t.ndim == 1:
+        t = t[:, None]
+    tab = np.zeros((t.shape[0], 20), np.float32)
+    tab[:, : t.shape[1]] = t
+    tab = np.ascontiguousarray(tab.astype(dt))
+    src_xyz = np.ascontiguousarray(src_xyz, np.int32).reshape(-1, 3)
+    src_kind = np.ascontiguousarray(src_kind, np.int32).reshape(-1)
+    rec_xyz = np.ascontiguousarray(rec_xyz, np.int32).reshape(-1, 3)
+    n_src, n_rec = src_xyz.shape[0], rec_xyz.shape[0]
+    sig = np.zeros((n_src, n_steps), dt)
+    s = np.asarray(signals).reshape(n_src, -1)
+    k = min(n_steps, s.shape[1])
+    sig[:, :k] = s[:, :k].astype(dt)
+    sig = np.ascontiguousarray(sig)
+    resp = np.zeros((n_rec, n_steps), dt)
+    P = np.zeros((Z, Y, X), dt) if want_fields else None
+    Pp = np.zeros((Z, Y, X), dt) if want_fields else None
+    if precision == "f64":
+        rc = lib().orc_run_d(_ptr(pos), _ptr(mat), X, Y, Z, _ptr(tab), tab.shape[0], octave, lam,
+                             _ptr(src_xyz), _ptr(src_kind), _ptr(sig), n_src, _ptr(rec_xyz),
+                             n_rec, n_steps, literal, threads, _ptr(resp), _ptr(P), _ptr(Pp))
+    else:
+        rc = lib().orc_run_f(_ptr(pos), _ptr(mat), X, Y, Z, _ptr(tab), tab.shape[0], octave, lam,
+                             _ptr(src_xyz), _ptr(src_kind), _ptr(sig), n_src, _ptr(rec_xyz),
+                             n_rec, n_steps, literal, storage, scale_exp, threads, _ptr(resp),
+                             _ptr(P), _ptr(Pp))
+    if rc != 0:
+        raise ValueError(f"oracle run failed with {rc}")
+    return (resp, P, Pp) if want_fields else resp

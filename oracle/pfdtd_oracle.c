@@ -1,0 +1,565 @@
+/*
+ * pfdtd_oracle.c -- CPU restatement of ParallelFDTD's SRL_FORWARD time-stepping core.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity oracle). Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load this library, always as the checker or the timed CPU
+ * baseline, never as a product path. See pfdtd_oracle.h for the per-function pins.
+ *
+ * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include "pfdtd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_PI 3.14159265358979323846 /* src/math/geomMath.h:68 */
+#define ORC_COEF_NUM 20               /* MATERIAL_COEF_NUM, src/base/MaterialHandler.h:29 */
+#define ORC_POS_MASK 0x7F             /* FORWARD_POSITION_MASK, src/kernels/cudaMesh.h:36 */
+#define ORC_INSIDE_SHIFT 7            /* INSIDE_SWITCH, src/kernels/cudaMesh.h:35 */
+
+int orc_version(void) { return 1; }
+
+/* ------------------------------------------------------------------------------------ */
+/* Classification                                                                        */
+/* ------------------------------------------------------------------------------------ */
+
+/* toBilbao, src/kernels/cudaMesh.cu:328-361 */
+uint8_t orc_to_bilbao(uint8_t k, uint8_t *mat_inout) {
+  if (k == 0) {
+    if (mat_inout) *mat_inout = 0;
+    return 0;
+  }
+  if (k <= 8) return (uint8_t)(3 | 0x80);
+  if (k <= 20) return (uint8_t)(4 | 0x80);
+  if (k <= 26) return (uint8_t)(5 | 0x80);
+  if (k == 27) return (uint8_t)(6 | 0x80);
+  return k; /* codes above 27 fall through every branch and are left unchanged */
+}
+
+void orc_classify_labels(const uint8_t *label, int64_t X, int64_t Y, int64_t Z, uint8_t *pos,
+                         uint8_t *mat, uint64_t counts[9]) {
+  const int64_t XY = X * Y;
+  uint64_t c[9];
+  memset(c, 0, sizeof(c));
+  for (int64_t z = 0; z < Z; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      for (int64_t x = 0; x < X; ++x) {
+        const int64_t i = z * XY + y * X + x;
+        if (label[i] != 0) {
+          pos[i] = 0;
+          mat[i] = 0;
+          continue;
+        }
+        /* neighbour order -x,+x,-y,+y,-z,+z; out-of-grid neighbours are solid material 0 */
+        const int inb[6] = {x > 0, x < X - 1, y > 0, y < Y - 1, z > 0, z < Z - 1};
+        const int64_t off[6] = {-1, 1, -X, X, -XY, XY};
+        int K = 0, found = 0;
+        unsigned m = 0;
+        for (int d = 0; d < 6; ++d) {
+          const unsigned lab = inb[d] ? label[i + off[d]] : 1u;
+          if (lab == 0) {
+            ++K;
+          } else if (!found) {
+            found = 1;
+            m = inb[d] ? lab - 1u : 0u;
+          }
+        }
+        pos[i] = (uint8_t)(0x80 | K);
+        mat[i] = (uint8_t)(K < 6 ? m : 0);
+        c[2 + K]++;
+        if (pos[i] == 0x86)
+          c[0]++;
+        else
+          c[1]++;
+      }
+    }
+  }
+  if (counts) memcpy(counts, c, sizeof(c));
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Parameters, materials, coordinates, source signals                                   */
+/* ------------------------------------------------------------------------------------ */
+
+/* SimulationParameters::getParameterPtr, SimulationParameters.cpp:379-387 */
+void orc_param_vector_f(double lambda, unsigned octave, float out[4]) {
+  out[0] = (float)lambda;
+  out[1] = (float)(lambda * lambda);
+  out[2] = 1.f / 3.f;
+  out[3] = (float)octave;
+}
+
+/* SimulationParameters::getParameterPtrDouble, SimulationParameters.cpp:389-396 */
+void orc_param_vector_d(double lambda, unsigned octave, double out[4]) {
+  out[0] = lambda;
+  out[1] = lambda * lambda;
+  out[2] = (double)1 / (double)3;
+  out[3] = (double)octave;
+}
+
+/* reflection2Admitance, src/global_includes.h:30-33 */
+float orc_r2y_f(float r) { return (1.f - r) / (1.f + r); }
+
+/* SimulationParameters::getDx, SimulationParameters.cpp:362-364 */
+float orc_dx(float c, unsigned fs, double lambda) {
+  return (float)((double)c / ((double)fs * lambda));
+}
+
+/* Position::getElementIdx, SrcRec.cpp:26-34; nv::ROUND, geomMath.h:126-128;
+ * padding +1, SimulationParameters.cpp:200-224 */
+void orc_element_idx(float px, float py, float pz, unsigned fs, float c, double lambda,
+                     int add_padding, int32_t out[3]) {
+  const float lam = (float)lambda;
+  const float dx = c / ((float)fs * lam);
+  out[0] = (int32_t)floorf(px / dx + 0.5f);
+  out[1] = (int32_t)floorf(py / dx + 0.5f);
+  out[2] = (int32_t)floorf(pz / dx + 0.5f);
+  if (add_padding) {
+    out[0] += 1;
+    out[1] += 1;
+    out[2] += 1;
+  }
+}
+
+/* getRegularSourceSample, SimulationParameters.cpp:262-298 */
+float orc_source_sample_f(int input_type, unsigned step, unsigned fs, const float *data,
+                          unsigned n_data) {
+  float sample = 0.f;
+  switch (input_type) {
+  case ORC_IN_IMPULSE:
+    sample += (step == 1) ? 1.f : 0.f;
+    break;
+  case ORC_IN_GAUSSIAN: {
+    const float t0 = 40;
+    const float width = 4;
+    const float e = ((float)step - t0) / width;
+    sample += expf(-0.5f * (e * e));
+    break;
+  }
+  case ORC_IN_DATA:
+    sample += (data && step < n_data) ? data[step] : 0.f;
+    break;
+  case ORC_IN_SINE: {
+    const float freq = 120;
+    const float t = (float)step / (float)fs;
+    sample += sinf(2.f * (float)ORC_PI * freq * t);
+    break;
+  }
+  default:
+    break;
+  }
+  return sample;
+}
+
+/* getRegularSourceSampleDouble, SimulationParameters.cpp:312-348 */
+double orc_source_sample_d(int input_type, unsigned step, unsigned fs, const double *data,
+                           unsigned n_data) {
+  double sample = 0.0;
+  switch (input_type) {
+  case ORC_IN_IMPULSE:
+    sample += (step == 1) ? 1.0 : 0.0;
+    break;
+  case ORC_IN_GAUSSIAN: {
+    const double t0 = 40;
+    const double width = 4;
+    const double e = ((float)((double)step - t0) / width);
+    sample += exp(-0.5f * (e * e));
+    break;
+  }
+  case ORC_IN_DATA:
+    sample += (data && step < n_data) ? data[step] : 0.0;
+    break;
+  case ORC_IN_SINE: {
+    const double freq = 120;
+    const double t = (double)step / (double)fs;
+    sample += sin(2.f * (double)ORC_PI * freq * t);
+    break;
+  }
+  default:
+    break;
+  }
+  return sample;
+}
+
+/* getGridIrDataSample, SimulationParameters.cpp:191-198: 0 past the end */
+static float grid_ir_at(const float *h, unsigned n, unsigned i) { return (h && i < n) ? h[i] : 0.f; }
+
+/* getTransparentSourceSample, SimulationParameters.cpp:300-310 */
+float orc_transparent_sample_f(int input_type, unsigned step, unsigned fs, const float *data,
+                               unsigned n_data, const float *grid_ir, unsigned n_ir) {
+  float sample = 0.f;
+  for (unsigned i = 0; i < step; i++)
+    sample += grid_ir_at(grid_ir, n_ir, step - i) *
+              orc_source_sample_f(input_type, i, fs, data, n_data);
+  return orc_source_sample_f(input_type, step, fs, data, n_data) - sample;
+}
+
+/* getTransparentSourceSampleDouble, SimulationParameters.cpp:350-360 */
+double orc_transparent_sample_d(int input_type, unsigned step, unsigned fs, const double *data,
+                                unsigned n_data, const float *grid_ir, unsigned n_ir) {
+  double sample = 0.0;
+  for (unsigned i = 0; i < step; i++)
+    sample += grid_ir_at(grid_ir, n_ir, step - i) *
+              orc_source_sample_d(input_type, i, fs, data, n_data);
+  return orc_source_sample_d(input_type, step, fs, data, n_data) - sample;
+}
+
+/* kernels3d.cu:513 (literal, `mat*20*+params[3]`) vs kernels3d.cu:639 (intended) */
+int64_t orc_material_slot(unsigned m, unsigned octave, int literal) {
+  if (literal) {
+    const unsigned base = m * 20u;
+    return (int64_t)(unsigned)((float)base * (float)octave);
+  }
+  return (int64_t)m * ORC_COEF_NUM + (int64_t)octave;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Storage rounding (round to nearest even)                                             */
+/* ------------------------------------------------------------------------------------ */
+
+uint16_t orc_f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u));
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u); /* >= 65520 rounds to inf */
+  if (ax < 0x38800000u) {                                    /* below 2^-14: subnormal */
+    if (ax < 0x33000000u) return (uint16_t)sign;             /* below 2^-25 */
+    const uint32_t e = ax >> 23;
+    const uint32_t m = (ax & 0x7FFFFFu) | 0x800000u;
+    const uint32_t shift = 126u - e; /* value in units of 2^-24 = m >> shift */
+    uint32_t q = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u);
+    const uint32_t half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (q & 1u))) q++;
+    return (uint16_t)(sign | q);
+  }
+  const uint32_t e = (ax >> 23) - 127u + 15u;
+  const uint32_t m = ax & 0x7FFFFFu;
+  uint32_t q = (e << 10) | (m >> 13);
+  const uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+  return (uint16_t)(sign | q);
+}
+
+float orc_f16_to_f32(uint16_t h) {
+  const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+  const uint32_t e = ((uint32_t)h >> 10) & 0x1Fu;
+  const uint32_t m = (uint32_t)h & 0x3FFu;
+  uint32_t x;
+  if (e == 0) {
+    if (m == 0) {
+      x = sign;
+    } else {
+      const float v = (float)m * 5.9604644775390625e-08f; /* m * 2^-24, exact */
+      uint32_t vb;
+      memcpy(&vb, &v, 4);
+      x = sign | vb;
+    }
+  } else if (e == 31) {
+    x = sign | 0x7F800000u | (m << 13);
+  } else {
+    x = sign | ((e - 15u + 127u) << 23) | (m << 13);
+  }
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+uint16_t orc_f32_to_bf16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  if ((x & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((x >> 16) | 0x40u);
+  const uint32_t lsb = (x >> 16) & 1u;
+  x += 0x7FFFu + lsb;
+  return (uint16_t)(x >> 16);
+}
+
+float orc_bf16_to_f32(uint16_t h) {
+  const uint32_t x = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+static inline float store_round(float v, int storage) {
+  if (storage == ORC_STORE_F16) return orc_f16_to_f32(orc_f32_to_f16(v));
+  if (storage == ORC_STORE_BF16) return orc_bf16_to_f32(orc_f32_to_bf16(v));
+  return v;
+}
+
+/* ------------------------------------------------------------------------------------ */
+/* Step loop                                                                             */
+/* ------------------------------------------------------------------------------------ */
+
+static int shell_is_outside(const uint8_t *pos, int64_t X, int64_t Y, int64_t Z) {
+  const int64_t XY = X * Y;
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t y = 0; y < Y; ++y) {
+      const int edge_zy = (z == 0 || z == Z - 1 || y == 0 || y == Y - 1);
+      for (int64_t x = 0; x < X; ++x) {
+        if (!edge_zy && x != 0 && x != X - 1) continue;
+        if (pos[z * XY + y * X + x] >> ORC_INSIDE_SHIFT) return 0;
+      }
+    }
+  return 1;
+}
+
+static int in_grid(const int32_t *p, int64_t X, int64_t Y, int64_t Z) {
+  return p[0] >= 0 && p[1] >= 0 && p[2] >= 0 && p[0] < X && p[1] < Y && p[2] < Z;
+}
+
+static int64_t lin(const int32_t *p, int64_t X, int64_t Y) {
+  return ((int64_t)p[2] * Y + p[1]) * X + p[0];
+}
+
+static int check_common(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+                        int n_mat, unsigned octave, int literal, int n_src, int n_rec,
+                        int n_steps) {
+  if (!pos || !mat || X < 3 || Y < 3 || Z < 3 || n_mat < 1 || n_mat > 256 || n_src < 0 ||
+      n_rec < 0 || n_steps < 0)
+    return -1;
+  if (octave >= ORC_COEF_NUM) return -1;
+  /* every material byte present must address a slot inside the table */
+  int seen[256];
+  memset(seen, 0, sizeof(seen));
+  const int64_t N = X * Y * Z;
+  for (int64_t i = 0; i < N; ++i) seen[mat[i]] = 1;
+  for (int m = 0; m < 256; ++m)
+    if (seen[m] && orc_material_slot((unsigned)m, octave, literal & 1) >= (int64_t)n_mat * ORC_COEF_NUM)
+      return -1;
+  if (!shell_is_outside(pos, X, Y, Z)) return -2;
+  return 0;
+}
+
+/* One fdtd3dStdMaterials<float> sweep (kernels3d.cu:485-529) over slices z in [1, Z-2]. */
+static void sweep_f(const uint8_t *pos, const uint8_t *mat, const float *P, float *Pp,
+                    const float *beta_of_mat, float lam, float lam2, int64_t X, int64_t Y,
+                    int64_t Z, int storage, int threads) {
+  const int64_t XY = X * Y;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 1; z <= Z - 2; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      const int64_t row = z * XY + y * X;
+      if (y == 0 || y == Y - 1) {
+        for (int64_t x = 0; x < X; ++x) Pp[row + x] = 0.f;
+        continue;
+      }
+      Pp[row] = 0.f;
+      Pp[row + X - 1] = 0.f;
+      for (int64_t x = 1; x <= X - 2; ++x) {
+        const int64_t i = row + x;
+        const uint8_t pb = pos[i];
+        const float position = (float)(pb & ORC_POS_MASK);
+        const float switch_bit = (float)(pb >> ORC_INSIDE_SHIFT);
+        const float beta = (float)(0.5 * (double)(beta_of_mat[mat[i]] * (6.f - position) * lam));
+        const float p_zp = P[i + XY];
+        const float p_zm = P[i - XY];
+        const float p_yp = P[i + X];
+        const float p_ym = P[i - X];
+        const float p_xp = P[i + 1];
+        const float p_xm = P[i - 1];
+        const float p = P[i];
+        const float p_past = Pp[i];
+        const float S = p_zp + p_zm + p_yp + p_ym + p_xp + p_xm;
+        const float ret = switch_bit * (1.f / (1.f + beta)) *
+                          ((2.f - position * lam2) * p + lam2 * S - (1.f - beta) * p_past);
+        Pp[i] = store_round(ret, storage);
+      }
+    }
+  }
+}
+
+/* fdtd3dStdMaterials<double> sweep (kernels3d.cu:485-529). */
+static void sweep_d(const uint8_t *pos, const uint8_t *mat, const double *P, double *Pp,
+                    const double *beta_of_mat, double lam, double lam2, int64_t X, int64_t Y,
+                    int64_t Z, int threads) {
+  const int64_t XY = X * Y;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 1; z <= Z - 2; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      const int64_t row = z * XY + y * X;
+      if (y == 0 || y == Y - 1) {
+        for (int64_t x = 0; x < X; ++x) Pp[row + x] = 0.0;
+        continue;
+      }
+      Pp[row] = 0.0;
+      Pp[row + X - 1] = 0.0;
+      for (int64_t x = 1; x <= X - 2; ++x) {
+        const int64_t i = row + x;
+        const uint8_t pb = pos[i];
+        const double position = (double)(pb & ORC_POS_MASK);
+        const double switch_bit = (double)(pb >> ORC_INSIDE_SHIFT);
+        const double beta = 0.5 * (beta_of_mat[mat[i]] * (6.f - position) * lam);
+        const double p_zp = P[i + XY];
+        const double p_zm = P[i - XY];
+        const double p_yp = P[i + X];
+        const double p_ym = P[i - X];
+        const double p_xp = P[i + 1];
+        const double p_xm = P[i - 1];
+        const double p = P[i];
+        const double p_past = Pp[i];
+        const double S = p_zp + p_zm + p_yp + p_ym + p_xp + p_xm;
+        const double ret = switch_bit * (1.f / (1.f + beta)) *
+                           ((2.f - position * lam2) * p + lam2 * S - (1.f - beta) * p_past);
+        Pp[i] = ret;
+      }
+    }
+  }
+}
+
+int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+              const float *mat_table, int n_mat, unsigned octave, double lambda,
+              const int32_t *src_xyz, const int32_t *src_type, const float *signals, int n_src,
+              const int32_t *rec_xyz, int n_rec, int n_steps, int literal, int storage,
+              int scale_exp, int threads, float *responses, float *P_out, float *Pp_out) {
+  int rc = check_common(pos, mat, X, Y, Z, n_mat, octave, literal, n_src, n_rec, n_steps);
+  if (rc) return rc;
+  if (!mat_table || storage < 0 || storage > 2 || scale_exp < -100 || scale_exp > 100) return -1;
+  if (threads < 1) threads = 1;
+  float params[4];
+  orc_param_vector_f(lambda, octave, params);
+  float beta_of_mat[256];
+  for (int m = 0; m < 256; ++m) {
+    const int64_t slot = orc_material_slot((unsigned)m, octave, literal & 1);
+    beta_of_mat[m] = slot < (int64_t)n_mat * ORC_COEF_NUM ? mat_table[slot] : 0.f;
+  }
+  const int64_t N = X * Y * Z;
+  float *A = (float *)calloc((size_t)N, sizeof(float));
+  float *B = (float *)calloc((size_t)N, sizeof(float));
+  if (!A || !B) {
+    free(A);
+    free(B);
+    return -1;
+  }
+  float *P = A, *Pp = B;
+  const float scale = ldexpf(1.f, scale_exp);
+  const float inv_scale = ldexpf(1.f, -scale_exp);
+  if (responses) memset(responses, 0, sizeof(float) * (size_t)n_rec * (size_t)n_steps);
+  for (int n = 0; n < n_steps; ++n) {
+    /* source loop, kernels3d.cu:93-104 */
+    for (int s = 0; s < n_src; ++s) {
+      const int32_t *q = src_xyz + 3 * s;
+      if (!in_grid(q, X, Y, Z)) continue;
+      const int64_t i = lin(q, X, Y);
+      const float smp = signals[(size_t)s * (size_t)n_steps + (size_t)n] * scale;
+      if (src_type[s] == ORC_SRC_HARD || (literal & 2))
+        P[i] = store_round(smp, storage);
+      else
+        P[i] = store_round(P[i] + smp, storage);
+    }
+    sweep_f(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, storage, threads);
+    float *t = P; /* flipPressurePointers, cudaMesh.h:766-779 */
+    P = Pp;
+    Pp = t;
+    /* receiver loop, kernels3d.cu:164-173 */
+    for (int r = 0; r < n_rec; ++r) {
+      const int32_t *q = rec_xyz + 3 * r;
+      if (!in_grid(q, X, Y, Z) || !responses) continue;
+      responses[(size_t)r * (size_t)n_steps + (size_t)n] = P[lin(q, X, Y)] * inv_scale;
+    }
+  }
+  if (P_out) memcpy(P_out, P, sizeof(float) * (size_t)N);
+  if (Pp_out) memcpy(Pp_out, Pp, sizeof(float) * (size_t)N);
+  free(A);
+  free(B);
+  return 0;
+}
+
+int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+              const double *mat_table, int n_mat, unsigned octave, double lambda,
+              const int32_t *src_xyz, const int32_t *src_type, const double *signals, int n_src,
+              const int32_t *rec_xyz, int n_rec, int n_steps, int literal, int threads,
+              double *responses, double *P_out, double *Pp_out) {
+  int rc = check_common(pos, mat, X, Y, Z, n_mat, octave, literal, n_src, n_rec, n_steps);
+  if (rc) return rc;
+  if (!mat_table) return -1;
+  if (threads < 1) threads = 1;
+  double params[4];
+  orc_param_vector_d(lambda, octave, params);
+  double beta_of_mat[256];
+  for (int m = 0; m < 256; ++m) {
+    const int64_t slot = orc_material_slot((unsigned)m, octave, literal & 1);
+    beta_of_mat[m] = slot < (int64_t)n_mat * ORC_COEF_NUM ? mat_table[slot] : 0.0;
+  }
+  const int64_t N = X * Y * Z;
+  double *A = (double *)calloc((size_t)N, sizeof(double));
+  double *B = (double *)calloc((size_t)N, sizeof(double));
+  if (!A || !B) {
+    free(A);
+    free(B);
+    return -1;
+  }
+  double *P = A, *Pp = B;
+  if (responses) memset(responses, 0, sizeof(double) * (size_t)n_rec * (size_t)n_steps);
+  for (int n = 0; n < n_steps; ++n) {
+    for (int s = 0; s < n_src; ++s) {
+      const int32_t *q = src_xyz + 3 * s;
+      if (!in_grid(q, X, Y, Z)) continue;
+      const int64_t i = lin(q, X, Y);
+      const double smp = signals[(size_t)s * (size_t)n_steps + (size_t)n];
+      if (src_type[s] == ORC_SRC_HARD || (literal & 2))
+        P[i] = smp;
+      else
+        P[i] = P[i] + smp;
+    }
+    sweep_d(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, threads);
+    double *t = P;
+    P = Pp;
+    Pp = t;
+    for (int r = 0; r < n_rec; ++r) {
+      const int32_t *q = rec_xyz + 3 * r;
+      if (!in_grid(q, X, Y, Z) || !responses) continue;
+      responses[(size_t)r * (size_t)n_steps + (size_t)n] = P[lin(q, X, Y)];
+    }
+  }
+  if (P_out) memcpy(P_out, P, sizeof(double) * (size_t)N);
+  if (Pp_out) memcpy(Pp_out, Pp, sizeof(double) * (size_t)N);
+  free(A);
+  free(B);
+  return 0;
+}
+
+/* captureSliceFast / captureSliceKernel, visualizationUtils.cu:130-254, :500-527 */
+int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y, int64_t Z,
+                        int64_t slice, int orientation, float *out_p, uint8_t *out_pos) {
+  const int64_t XY = X * Y;
+  if (slice < 0) return -1;
+  if (orientation == 0) {
+    if (slice >= Z) return -1;
+    for (int64_t y = 0; y < Y; ++y)
+      for (int64_t x = 0; x < X; ++x) {
+        out_p[y * X + x] = P[slice * XY + y * X + x];
+        out_pos[y * X + x] = pos[slice * XY + y * X + x];
+      }
+    return 0;
+  }
+  if (orientation == 1) {
+    if (slice >= Y) return -1;
+    for (int64_t z = 0; z < Z; ++z)
+      for (int64_t x = 0; x < X; ++x) {
+        out_p[z * X + x] = P[z * XY + slice * X + x];
+        out_pos[z * X + x] = pos[z * XY + slice * X + x];
+      }
+    return 0;
+  }
+  if (orientation == 2) {
+    if (slice >= X) return -1;
+    for (int64_t z = 0; z < Z; ++z)
+      for (int64_t y = 0; y < Y; ++y) {
+        out_p[z * Y + y] = P[z * XY + y * X + slice];
+        out_pos[z * Y + y] = pos[z * XY + y * X + slice];
+      }
+    return 0;
+  }
+  return -1;
+}

@@ -1,0 +1,154 @@
+/*
+ * pfdtd_oracle.h -- CPU restatement of ParallelFDTD's SRL_FORWARD time-stepping core.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the parity oracle: a plain-C restatement of the
+ * reference algorithm used by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py as the *checker*. The product library (parallelfdtd_amd/libpfdtd.so)
+ * never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it follows (paths relative to the
+ * reference repository root). Floating-point expressions keep the reference's operand
+ * order and are compiled with -ffp-contract=off so that each product and sum is a
+ * separately rounded IEEE operation.
+ *
+ * Pinning (what checks each function independently of this file's own formulas) is
+ * listed per function below and enforced by tests/test_oracle_*.py.
+ */
+#ifndef PFDTD_ORACLE_H
+#define PFDTD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Source kinds: enum SrcType, src/base/SrcRec.h:33 */
+#define ORC_SRC_HARD 0
+#define ORC_SRC_SOFT 1
+#define ORC_SRC_TRANSPARENT 2
+/* Input kinds: enum InputType, src/base/SrcRec.h:34 */
+#define ORC_IN_IMPULSE 0
+#define ORC_IN_GAUSSIAN 1
+#define ORC_IN_SINE 2
+#define ORC_IN_DATA 3
+
+/* Field storage for the FP32-arithmetic run (precision study, no reference counterpart). */
+#define ORC_STORE_F32 0
+#define ORC_STORE_F16 1
+#define ORC_STORE_BF16 2
+
+int orc_version(void);
+
+/* Voxelizer code k (0..27) -> Bilbao position byte, material byte zeroed for k==0.
+ * Follows toBilbao, src/kernels/cudaMesh.cu:328-361.
+ * Pinned by: the code->K groups documented in toKowalczyk, cudaMesh.cu:372-479 (1-8 three
+ * air directions, 9-20 four, 21-26 five, 27 six), checked exhaustively in tests. */
+uint8_t orc_to_bilbao(uint8_t k, uint8_t *mat_inout);
+
+/* Classify a label grid (label 0 = inside/air, label m+1 = solid voxel of material m).
+ * pos = 0x80|K for inside voxels (K = number of inside face neighbours, out-of-grid counts
+ * as solid), 0 for solid voxels. mat = material of the first solid face neighbour in the
+ * order -x,+x,-y,+y,-z,+z for inside voxels with K<6, else 0.
+ * counts[0] = air (pos==0x86), counts[1] = boundary (pos not 0 and not 0x86), following
+ * calcBoundaries, src/kernels/cudaMesh.cu:500-514; counts[2+K] = histogram of K (inside only).
+ * Pinned by: closed-form box-room counts, brute-force neighbour counting in numpy. */
+void orc_classify_labels(const uint8_t *label, int64_t X, int64_t Y, int64_t Z,
+                         uint8_t *pos, uint8_t *mat, uint64_t counts[9]);
+
+/* Parameter vector [lambda, lambda^2, 1/3, octave]:
+ * SimulationParameters::getParameterPtr(+Double), src/base/SimulationParameters.cpp:379-396.
+ * Pinned by: tests/SimulationParametersTest.cpp:80-97. */
+void orc_param_vector_f(double lambda, unsigned octave, float out[4]);
+void orc_param_vector_d(double lambda, unsigned octave, double out[4]);
+
+/* Reflectance <-> admittance, (1-R)/(1+R): src/global_includes.h:30-38.
+ * Pinned by: involution property and R=0 -> 1, R=1 -> 0. */
+float orc_r2y_f(float r);
+
+/* dx = c / (fs * lambda): SimulationParameters::getDx, SimulationParameters.cpp:362-364
+ * (double arithmetic, float result). Pinned by SimulationParametersTest.cpp:16-22. */
+float orc_dx(float c, unsigned fs, double lambda);
+
+/* Element index of a position in metres: Position::getElementIdx, src/base/SrcRec.cpp:26-34,
+ * with nv::ROUND = floor(x+0.5f) (src/math/geomMath.h:126-128) and the +1 padding offset of
+ * SimulationParameters::getSource/ReceiverElementCoordinates, SimulationParameters.cpp:200-224.
+ * Pinned by: hand-computed indices in tests. */
+void orc_element_idx(float px, float py, float pz, unsigned fs, float c, double lambda,
+                     int add_padding, int32_t out[3]);
+
+/* Regular source sample: SimulationParameters::getRegularSourceSample(+Double),
+ * SimulationParameters.cpp:262-298 and :312-348. DATA reads past the end return 0
+ * (getInputDataSample, :161-174). Pinned by: SimulationParametersTest.cpp:126-185. */
+float orc_source_sample_f(int input_type, unsigned step, unsigned fs, const float *data,
+                          unsigned n_data);
+double orc_source_sample_d(int input_type, unsigned step, unsigned fs, const double *data,
+                           unsigned n_data);
+
+/* Transparent source sample s(n) - sum_{i<n} h(n-i) s(i):
+ * getTransparentSourceSample(+Double), SimulationParameters.cpp:300-310, :350-360.
+ * Pinned by: SimulationParametersTest.cpp:196-207 (s_T(3)=1/3 with the grid IR h[2]=-1/3). */
+float orc_transparent_sample_f(int input_type, unsigned step, unsigned fs, const float *data,
+                               unsigned n_data, const float *grid_ir, unsigned n_ir);
+double orc_transparent_sample_d(int input_type, unsigned step, unsigned fs, const double *data,
+                                unsigned n_data, const float *grid_ir, unsigned n_ir);
+
+/* Index into the [material][20] coefficient table used by the update for material byte m.
+ * intended: m*20 + octave (the centred kernel's form, kernels3d.cu:639).
+ * literal:  (unsigned)((float)(m*20) * (float)octave), i.e. the forward kernel's
+ *           `mat*20*+params[3]`, kernels3d.cu:513.
+ * Pinned by: literal==intended at (octave 1, m) vs (octave 0, m); slot 0 at octave 0. */
+int64_t orc_material_slot(unsigned m, unsigned octave, int literal);
+
+/* Full step loop. Follows launchFDTD3d / launchFDTD3dDouble, src/kernels/kernels3d.cu:83-191
+ * (inject all sources into P, update, flip, gather receivers), with the per-node update of
+ * fdtd3dStdMaterials, kernels3d.cu:485-529, applied to every node of slices z in [1, Z-2]
+ * (launch coverage kernels3d.cu:112-123). Nodes with x or y on the grid edge are written 0;
+ * the reference writes switchBit*(...) there and switchBit is 0 under the precondition that
+ * the outer voxel layer is outside (checked: returns -2 otherwise).
+ *
+ * mat_table: n_mat*20 coefficients laid out [material][coef] (MaterialHandler.cpp:100-164).
+ * src_xyz / rec_xyz: grid indices (x,y,z) per source/receiver (already padded).
+ * src_type: ORC_SRC_*; signals: [n_src][n_steps] samples of the source signal (already
+ * transparent-filtered where applicable). Sources/receivers outside the grid are skipped
+ * (receivers then stay 0, as kernels3d.cu:168).
+ * literal: bit0 = Q1 material slot (see orc_material_slot), bit1 = Q2 soft source overwrites
+ * (cudaMesh.h:362-366) instead of adding.
+ * responses: [n_rec][n_steps], response[r][n] = P^{n+1}[rec] (post-flip read, kernels3d.cu:164-173).
+ * P_out / Pp_out (optional): the final current and past fields after the last flip.
+ * storage (f32 entry only): ORC_STORE_* with field values stored scaled by 2^scale_exp,
+ * rounded to nearest even in the storage format after every write.
+ * Returns 0 on success, -1 bad argument, -2 precondition violated.
+ *
+ * Pinned by: the free-field closed form h[n] = E[U_n(c)] (tiny grids vs lattice-walk sums),
+ * K=6 air reduction P_next = S/3 - P_past, energy conservation at beta=0, decay at beta>0,
+ * mirror symmetry, and brute-force golden fixtures from an independent numpy evaluation. */
+int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+              const float *mat_table, int n_mat, unsigned octave, double lambda,
+              const int32_t *src_xyz, const int32_t *src_type, const float *signals, int n_src,
+              const int32_t *rec_xyz, int n_rec, int n_steps, int literal, int storage,
+              int scale_exp, int threads, float *responses, float *P_out, float *Pp_out);
+
+int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+              const double *mat_table, int n_mat, unsigned octave, double lambda,
+              const int32_t *src_xyz, const int32_t *src_type, const double *signals, int n_src,
+              const int32_t *rec_xyz, int n_rec, int n_steps, int literal, int threads,
+              double *responses, double *P_out, double *Pp_out);
+
+/* Slice gather of a field and the position bytes: captureSliceFast / captureSliceKernel,
+ * src/kernels/visualizationUtils.cu:130-254, :500-527. orientation 0 = xy at z=slice
+ * (out[y*X+x]), 1 = xz at y=slice (out[z*X+x]), 2 = yz at x=slice (out[z*Y+y]).
+ * Returns -1 when the slice is out of range (the reference logs and skips). */
+int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y, int64_t Z,
+                        int64_t slice, int orientation, float *out_p, uint8_t *out_pos);
+
+/* Storage rounding helpers (round to nearest even), exposed for tests. */
+uint16_t orc_f32_to_f16(float f);
+float orc_f16_to_f32(uint16_t h);
+uint16_t orc_f32_to_bf16(float f);
+float orc_bf16_to_f32(uint16_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

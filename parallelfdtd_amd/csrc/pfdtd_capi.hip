@@ -1,0 +1,876 @@
+// pfdtd_capi.hip -- C ABI of the SRL_FORWARD core (declarations and reference citations in
+// include/pfdtd.h). Host code: context, device memory, coefficient tables, the step loop.
+#include "../../include/pfdtd.h"
+#include "pfdtd_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace pfdtd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess)                                                                    \
+      return fail(PFDTD_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e),        \
+                  __FILE__, __LINE__);                                                       \
+  } while (0)
+
+constexpr int kCoefNum = 20;   // MATERIAL_COEF_NUM, src/base/MaterialHandler.h:29
+constexpr int kProgressMod = 100;  // PROGRESS_MOD, src/kernels/kernels3d.h:36
+
+size_t elem_size(int precision) {
+  switch (precision) {
+  case PFDTD_F32:
+    return 4;
+  case PFDTD_F64:
+    return 8;
+  case PFDTD_F16:
+  case PFDTD_BF16:
+    return 2;
+  default:
+    return 0;
+  }
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct pfdtd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // domain
+  int64_t X = 0, Y = 0, Z = 0, N = 0;
+  uint8_t *d_pos = nullptr, *d_mat = nullptr;
+  int64_t dom_cap = 0;  // voxels the position/material buffers hold
+  bool has_domain = false;
+  DomainScan scan{};
+  DomainScan *d_scan = nullptr;
+
+  // materials / parameters
+  bool has_materials = false;
+  std::vector<float> table;  // [n_mat][20]
+  int n_mat = 0;
+  unsigned octave = 0;
+  double lambda = 0.0;
+  int flags = 0;
+  CoefF *d_coef_f = nullptr;
+  CoefD *d_coef_d = nullptr;
+  float lam2_f = 0.f;
+  double lam2_d = 0.0;
+  CoefF air_f{};          // K = 6 row entry shared by all air voxels
+  CoefD air_d{};
+  int air_uniform = 0;    // 1 when every material gives the same K = 6 entry (both tables)
+  int ry = 2;             // y rows per thread of the update kernel
+  int zc = 8;             // slices per z-chunk of the update kernel
+
+  // sources / receivers
+  int n_src = 0, n_samples = 0;
+  std::vector<int32_t> src_xyz, src_kind;
+  int64_t *d_src_idx = nullptr;
+  int32_t *d_src_kind = nullptr;
+  float *d_sig_f = nullptr;
+  double *d_sig_d = nullptr;
+  int n_rec = 0;
+  std::vector<int32_t> rec_xyz;
+  int64_t *d_rec_idx = nullptr;
+
+  // fields
+  int precision = -1;
+  int scale_exp = 0;      // scale of the allocated fields
+  int scale_req = 0;      // requested F16/BF16 storage scale (pfdtd_prepare_fields)
+  void *d_P = nullptr, *d_Pp = nullptr;
+  size_t field_bytes = 0;
+
+  // responses
+  double *d_resp = nullptr;
+  int resp_steps = 0;
+  size_t resp_cap = 0;
+
+  // step-by-step state
+  int last_direction = 1;
+
+  // options / timing
+  int poll_interval = kProgressMod;
+  bool kernel_timing = false;
+  double upd_ms = 0.0;
+  int64_t upd_launches = 0;
+  std::vector<hipEvent_t> events;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+};
+
+namespace {
+
+int check_ctx(pfdtd_ctx *ctx) {
+  if (!ctx) return fail(PFDTD_E_ARG, "null context");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess)
+    return fail(PFDTD_E_HIP, "hipSetDevice(%d): %s", ctx->device, hipGetErrorString(e));
+  return PFDTD_OK;
+}
+
+int64_t lin_idx(const pfdtd_ctx *c, const int32_t *q) {
+  if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= c->X || q[1] >= c->Y || q[2] >= c->Z)
+    return -1;
+  return ((int64_t)q[2] * c->Y + q[1]) * c->X + q[0];
+}
+
+// kernels3d.cu:513 (literal `mat*20*+params[3]`) vs kernels3d.cu:639 (intended)
+int64_t material_slot(unsigned m, unsigned octave, bool literal) {
+  if (literal) {
+    const unsigned base = m * 20u;
+    return (int64_t)(unsigned)((float)base * (float)octave);
+  }
+  return (int64_t)m * kCoefNum + (int64_t)octave;
+}
+
+// The per-(K, m) coefficients, evaluated in the operation order of the reference update body
+// (kernels3d.cu:513-526) so that a[K,m] multiplies exactly like the per-node expression.
+void build_coefficients(const pfdtd_ctx *c, std::vector<CoefF> &cf, std::vector<CoefD> &cd) {
+#pragma clang fp contract(off)
+  cf.assign(kCoefEntries, CoefF{0.f, 0.f, 0.f, 0.f});
+  cd.assign(kCoefEntries, CoefD{0.0, 0.0, 0.0, 0.0});
+  const float lam_f = (float)c->lambda;                       // params[0]
+  const float lam2_f = (float)(c->lambda * c->lambda);        // params[1]
+  const double lam_d = c->lambda;
+  const double lam2_d = c->lambda * c->lambda;
+  const bool literal = (c->flags & PFDTD_MATSLOT_LITERAL) != 0;
+  for (int K = 0; K <= 6; ++K) {
+    for (int m = 0; m < 256; ++m) {
+      const int64_t slot = material_slot((unsigned)m, c->octave, literal);
+      const float bm = slot < (int64_t)c->n_mat * kCoefNum ? c->table[(size_t)slot] : 0.f;
+      const float position = (float)K;
+      const float beta = (float)(0.5 * (double)(bm * (6.f - position) * lam_f));
+      CoefF &f = cf[(size_t)K * 256 + m];
+      f.inv = 1.f / (1.f + beta);
+      f.omb = 1.f - beta;
+      f.tk = 2.f - position * lam2_f;
+      const double bmd = (double)bm;
+      const double positiond = (double)K;
+      const double betad = 0.5 * (bmd * (6.f - positiond) * lam_d);
+      CoefD &d = cd[(size_t)K * 256 + m];
+      d.inv = 1.f / (1.f + betad);
+      d.omb = 1.f - betad;
+      d.tk = 2.f - positiond * lam2_d;
+    }
+  }
+}
+
+int scan_domain(pfdtd_ctx *c) {
+  HIP_TRY(hipMemsetAsync(c->d_scan, 0, sizeof(DomainScan), c->stream));
+  HIP_TRY(launch_scan_domain(c->d_pos, c->d_mat, (int)c->X, (int)c->Y, (int)c->Z, c->d_scan,
+                             c->stream));
+  HIP_TRY(hipMemcpyAsync(&c->scan, c->d_scan, sizeof(DomainScan), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return PFDTD_OK;
+}
+
+int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
+  if (X < 3 || Y < 3 || Z < 3)
+    return fail(PFDTD_E_ARG, "grid %lldx%lldx%lld: every dimension must be >= 3",
+                (long long)X, (long long)Y, (long long)Z);
+  if (X > (1 << 30) || Y > 65535 || Z > 65535)
+    return fail(PFDTD_E_ARG, "grid %lldx%lldx%lld exceeds X<=2^30, Y<=65535, Z<=65535",
+                (long long)X, (long long)Y, (long long)Z);
+  const int64_t N = X * Y * Z;
+  if (c->d_pos && c->dom_cap != N) {
+    dfree(c->d_pos);
+    dfree(c->d_mat);
+    c->dom_cap = 0;
+  }
+  // fields sized for the previous grid are dropped
+  dfree(c->d_P);
+  dfree(c->d_Pp);
+  c->precision = -1;
+  c->field_bytes = 0;
+  if (!c->d_pos) {
+    hipError_t e1 = hipMalloc((void **)&c->d_pos, (size_t)N);
+    hipError_t e2 = e1 == hipSuccess ? hipMalloc((void **)&c->d_mat, (size_t)N) : e1;
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      dfree(c->d_pos);
+      dfree(c->d_mat);
+      (void)hipGetLastError();
+      return fail(PFDTD_E_NOMEM, "cannot allocate %lld-voxel domain bytes", (long long)N);
+    }
+    c->dom_cap = N;
+  }
+  c->X = X;
+  c->Y = Y;
+  c->Z = Z;
+  c->N = N;
+  c->has_domain = false;
+  return PFDTD_OK;
+}
+
+int finish_domain(pfdtd_ctx *c) {
+  int rc = scan_domain(c);
+  if (rc) return rc;
+  c->has_domain = true;
+  // source/receiver linear indices depend on the grid
+  if (c->n_src) {
+    std::vector<int64_t> idx(c->n_src);
+    for (int s = 0; s < c->n_src; ++s) idx[s] = lin_idx(c, &c->src_xyz[3 * s]);
+    HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src,
+                      hipMemcpyHostToDevice));
+  }
+  if (c->n_rec) {
+    std::vector<int64_t> idx(c->n_rec);
+    for (int r = 0; r < c->n_rec; ++r) idx[r] = lin_idx(c, &c->rec_xyz[3 * r]);
+    HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * c->n_rec,
+                      hipMemcpyHostToDevice));
+  }
+  return PFDTD_OK;
+}
+
+// Checks everything the update relies on before the first launch of a run.
+int check_runnable(pfdtd_ctx *c) {
+  if (!c->has_domain) return fail(PFDTD_E_STATE, "no domain set");
+  if (!c->has_materials) return fail(PFDTD_E_STATE, "no materials set");
+  if (c->scan.shell_inside)
+    return fail(PFDTD_E_DOMAIN,
+                "%u voxels of the outer layer are inside; the update requires an outside shell",
+                c->scan.shell_inside);
+  if (c->scan.bad_k)
+    return fail(PFDTD_E_DOMAIN, "%u inside voxels have K > 6 in their position byte",
+                c->scan.bad_k);
+  const bool literal = (c->flags & PFDTD_MATSLOT_LITERAL) != 0;
+  for (int m = 0; m < 256; ++m) {
+    if (!((c->scan.mat_present[m >> 5] >> (m & 31)) & 1u)) continue;
+    if (material_slot((unsigned)m, c->octave, literal) >= (int64_t)c->n_mat * kCoefNum)
+      return fail(PFDTD_E_DOMAIN,
+                  "material byte %d addresses coefficient slot %lld outside the %d-material "
+                  "table",
+                  m, (long long)material_slot((unsigned)m, c->octave, literal), c->n_mat);
+  }
+  return PFDTD_OK;
+}
+
+int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
+  const size_t es = elem_size(precision);
+  if (!es) return fail(PFDTD_E_ARG, "unknown precision %d", precision);
+  if ((precision == PFDTD_F32 || precision == PFDTD_F64) && scale_exp != 0)
+    return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
+  if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
+  const size_t bytes = es * (size_t)c->N;
+  if (c->d_P && c->field_bytes == bytes) {
+    c->precision = precision;
+    c->scale_exp = scale_exp;
+    return PFDTD_OK;
+  }
+  dfree(c->d_P);
+  dfree(c->d_Pp);
+  hipError_t e1 = hipMalloc(&c->d_P, bytes);
+  hipError_t e2 = e1 == hipSuccess ? hipMalloc(&c->d_Pp, bytes) : e1;
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    dfree(c->d_P);
+    dfree(c->d_Pp);
+    (void)hipGetLastError();
+    c->field_bytes = 0;
+    c->precision = -1;
+    return fail(PFDTD_E_NOMEM, "cannot allocate two %zu-byte pressure fields", bytes);
+  }
+  c->field_bytes = bytes;
+  c->precision = precision;
+  c->scale_exp = scale_exp;
+  return PFDTD_OK;
+}
+
+int zero_fields(pfdtd_ctx *c) {
+  HIP_TRY(hipMemsetAsync(c->d_P, 0, c->field_bytes, c->stream));
+  HIP_TRY(hipMemsetAsync(c->d_Pp, 0, c->field_bytes, c->stream));
+  c->last_direction = 1;
+  return PFDTD_OK;
+}
+
+int ensure_responses(pfdtd_ctx *c, int n_steps) {
+  const size_t need = (size_t)c->n_rec * (size_t)n_steps;
+  if (need > c->resp_cap) {
+    dfree(c->d_resp);
+    if (hipMalloc((void **)&c->d_resp, need * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      c->resp_cap = 0;
+      return fail(PFDTD_E_NOMEM, "cannot allocate responses");
+    }
+    c->resp_cap = need;
+  }
+  c->resp_steps = n_steps;
+  if (need) HIP_TRY(hipMemsetAsync(c->d_resp, 0, need * sizeof(double), c->stream));
+  return PFDTD_OK;
+}
+
+InjectArgs inject_args(const pfdtd_ctx *c, int gather_step, int inject_step) {
+  InjectArgs a;
+  a.rec_idx = c->d_rec_idx;
+  a.n_rec = c->n_rec;
+  a.resp = c->d_resp;
+  a.resp_steps = c->resp_steps;
+  a.gather_step = (c->n_rec > 0) ? gather_step : -1;
+  a.src_idx = c->d_src_idx;
+  a.src_kind = c->d_src_kind;
+  a.n_src = c->n_src;
+  a.sig_f = c->d_sig_f;
+  a.sig_d = c->d_sig_d;
+  a.n_samples = c->n_samples;
+  a.inject_step = (c->n_src > 0) ? inject_step : -1;
+  a.soft_overwrite = (c->flags & PFDTD_SOFT_OVERWRITE) ? 1 : 0;
+  a.scale_exp = c->scale_exp;
+  return a;
+}
+
+hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
+  switch (c->precision) {
+  case PFDTD_F32:
+    return launch_update_f32(g, UpdateArgs<float, CoefF, float>{c->d_pos, c->d_mat,
+                             (const float *)c->d_P, (float *)c->d_Pp, c->d_coef_f, c->air_f,
+                             c->air_uniform, c->lam2_f}, c->stream);
+  case PFDTD_F64:
+    return launch_update_f64(g, UpdateArgs<double, CoefD, double>{c->d_pos, c->d_mat,
+                             (const double *)c->d_P, (double *)c->d_Pp, c->d_coef_d, c->air_d,
+                             c->air_uniform, c->lam2_d}, c->stream);
+  case PFDTD_F16:
+    return launch_update_f16(g, UpdateArgs<h16, CoefF, float>{c->d_pos, c->d_mat,
+                             (const h16 *)c->d_P, (h16 *)c->d_Pp, c->d_coef_f, c->air_f,
+                             c->air_uniform, c->lam2_f}, c->stream);
+  case PFDTD_BF16:
+    return launch_update_bf16(g, UpdateArgs<b16, CoefF, float>{c->d_pos, c->d_mat,
+                              (const b16 *)c->d_P, (b16 *)c->d_Pp, c->d_coef_f, c->air_f,
+                              c->air_uniform, c->lam2_f}, c->stream);
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+int ensure_events(pfdtd_ctx *c, size_t n) {
+  while (c->events.size() < n) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    c->events.push_back(e);
+  }
+  return PFDTD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pfdtd_last_error(void) { return g_last_error.c_str(); }
+
+int pfdtd_version(void) { return 1; }
+
+int pfdtd_device_count(int *count) {
+  if (!count) return fail(PFDTD_E_ARG, "null count");
+  HIP_TRY(hipGetDeviceCount(count));
+  return PFDTD_OK;
+}
+
+int pfdtd_device_mem(int device, uint64_t *free_bytes, uint64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(PFDTD_E_ARG, "null output");
+  HIP_TRY(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  *free_bytes = f;
+  *total_bytes = t;
+  return PFDTD_OK;
+}
+
+int pfdtd_ctx_create(int device, pfdtd_ctx **out) {
+  if (!out) return fail(PFDTD_E_ARG, "null output pointer");
+  *out = nullptr;
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(PFDTD_E_ARG, "device %d not present (%d devices)", device, n);
+  HIP_TRY(hipSetDevice(device));
+  pfdtd_ctx *c = new pfdtd_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreate(&c->stream);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_scan, sizeof(DomainScan));
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_coef_f, sizeof(CoefF) * kCoefEntries);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_coef_d, sizeof(CoefD) * kCoefEntries);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev_begin);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev_end);
+  if (e != hipSuccess) {
+    pfdtd_ctx_destroy(c);
+    return fail(PFDTD_E_HIP, "context setup on device %d: %s", device, hipGetErrorString(e));
+  }
+  *out = c;
+  return PFDTD_OK;
+}
+
+int pfdtd_ctx_destroy(pfdtd_ctx *c) {
+  if (!c) return PFDTD_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dfree(c->d_pos);
+  dfree(c->d_mat);
+  dfree(c->d_scan);
+  dfree(c->d_coef_f);
+  dfree(c->d_coef_d);
+  dfree(c->d_src_idx);
+  dfree(c->d_src_kind);
+  dfree(c->d_sig_f);
+  dfree(c->d_sig_d);
+  dfree(c->d_rec_idx);
+  dfree(c->d_P);
+  dfree(c->d_Pp);
+  dfree(c->d_resp);
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+  if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
+  if (c->ev_end) (void)hipEventDestroy(c->ev_end);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_domain(pfdtd_ctx *c, const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y,
+                     int64_t Z) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!pos || !mat) return fail(PFDTD_E_ARG, "null position or material array");
+  rc = alloc_domain(c, X, Y, Z);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(c->d_pos, pos, (size_t)c->N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_mat, mat, (size_t)c->N, hipMemcpyHostToDevice));
+  return finish_domain(c);
+}
+
+int pfdtd_set_domain_labels(pfdtd_ctx *c, const uint8_t *labels, int64_t X, int64_t Y,
+                            int64_t Z) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!labels) return fail(PFDTD_E_ARG, "null label array");
+  rc = alloc_domain(c, X, Y, Z);
+  if (rc) return rc;
+  uint8_t *d_lab = nullptr;
+  if (hipMalloc((void **)&d_lab, (size_t)c->N) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PFDTD_E_NOMEM, "cannot allocate the label grid");
+  }
+  hipError_t e = hipMemcpy(d_lab, labels, (size_t)c->N, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_classify_labels(d_lab, c->d_pos, c->d_mat, (int)X, (int)Y, (int)Z, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_lab);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "label classification: %s", hipGetErrorString(e));
+  return finish_domain(c);
+}
+
+int pfdtd_set_domain_codes(pfdtd_ctx *c, const uint8_t *codes, const uint8_t *mat, int64_t X,
+                           int64_t Y, int64_t Z) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!codes || !mat) return fail(PFDTD_E_ARG, "null code or material array");
+  rc = alloc_domain(c, X, Y, Z);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(c->d_pos, codes, (size_t)c->N, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_mat, mat, (size_t)c->N, hipMemcpyHostToDevice));
+  HIP_TRY(launch_codes_to_bilbao(c->d_pos, c->d_mat, c->N, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return finish_domain(c);
+}
+
+int pfdtd_get_domain(pfdtd_ctx *c, uint8_t *pos, uint8_t *mat) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->has_domain) return fail(PFDTD_E_STATE, "no domain set");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (pos) HIP_TRY(hipMemcpy(pos, c->d_pos, (size_t)c->N, hipMemcpyDeviceToHost));
+  if (mat) HIP_TRY(hipMemcpy(mat, c->d_mat, (size_t)c->N, hipMemcpyDeviceToHost));
+  return PFDTD_OK;
+}
+
+int pfdtd_get_dims(pfdtd_ctx *c, int64_t *X, int64_t *Y, int64_t *Z) {
+  if (!c || !X || !Y || !Z) return fail(PFDTD_E_ARG, "null argument");
+  *X = c->X;
+  *Y = c->Y;
+  *Z = c->Z;
+  return PFDTD_OK;
+}
+
+int pfdtd_get_counts(pfdtd_ctx *c, uint64_t *air, uint64_t *boundary) {
+  if (!c || !air || !boundary) return fail(PFDTD_E_ARG, "null argument");
+  if (!c->has_domain) return fail(PFDTD_E_STATE, "no domain set");
+  *air = c->scan.air;
+  *boundary = c->scan.boundary;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_materials(pfdtd_ctx *c, const float *table, int n_mat, int n_coef,
+                        unsigned octave, double lambda, int flags) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!table) return fail(PFDTD_E_ARG, "null material table");
+  if (n_mat < 1 || n_mat > 256) return fail(PFDTD_E_ARG, "n_mat %d outside 1..256", n_mat);
+  if (n_coef < 1 || n_coef > kCoefNum)
+    return fail(PFDTD_E_ARG, "n_coef %d outside 1..%d", n_coef, kCoefNum);
+  if (octave >= (unsigned)kCoefNum) return fail(PFDTD_E_ARG, "octave %u >= 20", octave);
+  if (!(lambda > 0.0) || !std::isfinite(lambda)) return fail(PFDTD_E_ARG, "bad lambda");
+  if (flags & ~(PFDTD_MATSLOT_LITERAL | PFDTD_SOFT_OVERWRITE))
+    return fail(PFDTD_E_ARG, "unknown flags 0x%x", flags);
+  c->table.assign((size_t)n_mat * kCoefNum, 0.f);
+  for (int m = 0; m < n_mat; ++m)
+    for (int j = 0; j < n_coef; ++j) c->table[(size_t)m * kCoefNum + j] = table[(size_t)m * n_coef + j];
+  c->n_mat = n_mat;
+  c->octave = octave;
+  c->lambda = lambda;
+  c->flags = flags;
+  c->lam2_f = (float)(lambda * lambda);
+  c->lam2_d = lambda * lambda;
+  std::vector<CoefF> cf;
+  std::vector<CoefD> cd;
+  build_coefficients(c, cf, cd);
+  c->air_f = cf[6 * 256];
+  c->air_d = cd[6 * 256];
+  c->air_uniform = 1;
+  for (int m = 0; m < 256; ++m) {
+    if (std::memcmp(&cf[6 * 256 + m], &c->air_f, sizeof(CoefF)) != 0 ||
+        std::memcmp(&cd[6 * 256 + m], &c->air_d, sizeof(CoefD)) != 0)
+      c->air_uniform = 0;
+  }
+  HIP_TRY(hipMemcpy(c->d_coef_f, cf.data(), sizeof(CoefF) * cf.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_coef_d, cd.data(), sizeof(CoefD) * cd.size(), hipMemcpyHostToDevice));
+  c->has_materials = true;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t *kind,
+                      int n_samples, const float *signal_f32, const double *signal_f64) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_src < 0 || n_samples < 0) return fail(PFDTD_E_ARG, "negative count");
+  if (n_src > 0 && (!xyz || !kind)) return fail(PFDTD_E_ARG, "null source arrays");
+  if (n_src > 0 && n_samples > 0 && !signal_f32 && !signal_f64)
+    return fail(PFDTD_E_ARG, "no source signal");
+  for (int s = 0; s < n_src; ++s)
+    if (kind[s] < PFDTD_SRC_HARD || kind[s] > PFDTD_SRC_TRANSPARENT)
+      return fail(PFDTD_E_ARG, "source %d: unknown kind %d", s, kind[s]);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  dfree(c->d_src_idx);
+  dfree(c->d_src_kind);
+  dfree(c->d_sig_f);
+  dfree(c->d_sig_d);
+  c->n_src = 0;
+  c->n_samples = 0;
+  c->src_xyz.assign(xyz, xyz + 3 * (size_t)n_src);
+  c->src_kind.assign(kind, kind + n_src);
+  if (n_src == 0) return PFDTD_OK;
+  const size_t ns = (size_t)n_src * (size_t)n_samples;
+  std::vector<float> sf(ns);
+  std::vector<double> sd(ns);
+  for (size_t i = 0; i < ns; ++i) {
+    sf[i] = signal_f32 ? signal_f32[i] : (float)signal_f64[i];
+    sd[i] = signal_f64 ? signal_f64[i] : (double)signal_f32[i];
+  }
+  std::vector<int64_t> idx(n_src, -1);
+  if (c->has_domain)
+    for (int s = 0; s < n_src; ++s) idx[s] = lin_idx(c, &c->src_xyz[3 * s]);
+  HIP_TRY(hipMalloc((void **)&c->d_src_idx, sizeof(int64_t) * n_src));
+  HIP_TRY(hipMalloc((void **)&c->d_src_kind, sizeof(int32_t) * n_src));
+  HIP_TRY(hipMalloc((void **)&c->d_sig_f, sizeof(float) * (ns ? ns : 1)));
+  HIP_TRY(hipMalloc((void **)&c->d_sig_d, sizeof(double) * (ns ? ns : 1)));
+  HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * n_src, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_src_kind, kind, sizeof(int32_t) * n_src, hipMemcpyHostToDevice));
+  if (ns) {
+    HIP_TRY(hipMemcpy(c->d_sig_f, sf.data(), sizeof(float) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_sig_d, sd.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+  }
+  c->n_src = n_src;
+  c->n_samples = n_samples;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_receivers(pfdtd_ctx *c, int n_rec, const int32_t *xyz) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_rec < 0) return fail(PFDTD_E_ARG, "negative receiver count");
+  if (n_rec > 0 && !xyz) return fail(PFDTD_E_ARG, "null receiver array");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  dfree(c->d_rec_idx);
+  c->n_rec = 0;
+  c->rec_xyz.assign(xyz, xyz + 3 * (size_t)n_rec);
+  c->resp_steps = 0;
+  if (n_rec == 0) return PFDTD_OK;
+  std::vector<int64_t> idx(n_rec, -1);
+  if (c->has_domain)
+    for (int r = 0; r < n_rec; ++r) idx[r] = lin_idx(c, &c->rec_xyz[3 * r]);
+  HIP_TRY(hipMalloc((void **)&c->d_rec_idx, sizeof(int64_t) * n_rec));
+  HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * n_rec, hipMemcpyHostToDevice));
+  c->n_rec = n_rec;
+  return PFDTD_OK;
+}
+
+int pfdtd_prepare_fields(pfdtd_ctx *c, int precision, int scale_exp) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->has_domain) return fail(PFDTD_E_STATE, "no domain set");
+  rc = ensure_fields(c, precision, scale_exp);
+  if (rc) return rc;
+  if (precision == PFDTD_F16 || precision == PFDTD_BF16) c->scale_req = scale_exp;
+  return zero_fields(c);
+}
+
+int pfdtd_reset_fields(pfdtd_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->d_P) return fail(PFDTD_E_STATE, "fields not prepared");
+  return zero_fields(c);
+}
+
+int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb interrupt,
+              pfdtd_progress_cb progress, void *user, int *steps_done, double *ms_elapsed) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_steps < 0) return fail(PFDTD_E_ARG, "negative step count");
+  rc = check_runnable(c);
+  if (rc) return rc;
+  const int scale = (precision == PFDTD_F16 || precision == PFDTD_BF16) ? c->scale_req : 0;
+  rc = ensure_fields(c, precision, scale);
+  if (rc) return rc;
+  rc = zero_fields(c);
+  if (rc) return rc;
+  rc = ensure_responses(c, n_steps);
+  if (rc) return rc;
+  if (steps_done) *steps_done = 0;
+  c->upd_ms = 0.0;
+  c->upd_launches = 0;
+  const UpdateGeom g = plan_update((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc);
+  const int bundle = c->poll_interval > 0 ? c->poll_interval : kProgressMod;
+  if (c->kernel_timing) {
+    rc = ensure_events(c, 2 * (size_t)bundle);
+    if (rc) return rc;
+  }
+
+  HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
+  // step 0's source samples go into the (zero) current field before the first update
+  if (n_steps > 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, 0), c->stream));
+
+  int done = 0;
+  auto wall0 = std::chrono::steady_clock::now();
+  while (done < n_steps) {
+    if (interrupt && interrupt(user)) break;  // kernels3d.cu:86-89
+    const int nb = std::min(bundle, n_steps - done);
+    for (int k = 0; k < nb; ++k) {
+      const int step = done + k;
+      if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * k], c->stream));
+      HIP_TRY(launch_update_any(c, g));
+      if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * k + 1], c->stream));
+      // after the update the new field lives in d_Pp: gather this step, inject the next
+      const int next = step + 1 < n_steps ? step + 1 : -1;
+      HIP_TRY(launch_inject_gather(c->precision, c->d_Pp, inject_args(c, step, next), c->stream));
+      std::swap(c->d_P, c->d_Pp);  // flipPressurePointers, cudaMesh.h:766-779
+    }
+    if (interrupt || progress || c->kernel_timing) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if (c->kernel_timing) {
+        for (int k = 0; k < nb; ++k) {
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, c->events[2 * k], c->events[2 * k + 1]));
+          c->upd_ms += ms;
+        }
+        c->upd_launches += nb;
+      }
+    }
+    const int first = done;
+    done += nb;
+    if (progress) {
+      // PROGRESS_MOD steps: report at every multiple of 100 passed in this bundle
+      const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+      for (int s = ((first + kProgressMod - 1) / kProgressMod) * kProgressMod; s < done; s += kProgressMod)
+        progress(s, n_steps, (float)(sec / done), user);
+    }
+  }
+  HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (ms_elapsed) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_begin, c->ev_end));
+    *ms_elapsed = ms;
+  }
+  if (steps_done) *steps_done = done;
+  return PFDTD_OK;
+}
+
+int pfdtd_begin_steps(pfdtd_ctx *c, int precision, int n_steps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_steps < 0) return fail(PFDTD_E_ARG, "negative step count");
+  rc = check_runnable(c);
+  if (rc) return rc;
+  const int scale = (precision == PFDTD_F16 || precision == PFDTD_BF16) ? c->scale_req : 0;
+  rc = ensure_fields(c, precision, scale);
+  if (rc) return rc;
+  rc = zero_fields(c);
+  if (rc) return rc;
+  return ensure_responses(c, n_steps);
+}
+
+int pfdtd_step(pfdtd_ctx *c, int step, int direction) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->d_P || c->precision < 0) return fail(PFDTD_E_STATE, "fields not prepared");
+  if (direction != 1 && direction != -1) return fail(PFDTD_E_ARG, "direction must be +1 or -1");
+  rc = check_runnable(c);
+  if (rc) return rc;
+  // launchFDTD3dStep: inject this step's samples, update, flip unless the direction changed,
+  // then read receivers into response[step] (kernels3d.cu:390-475).
+  if (step >= 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, step), c->stream));
+  const UpdateGeom g = plan_update((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc);
+  HIP_TRY(launch_update_any(c, g));
+  if (c->last_direction == direction) std::swap(c->d_P, c->d_Pp);
+  c->last_direction = direction;
+  if (step >= 0 && step < c->resp_steps)
+    HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, step, -1), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return PFDTD_OK;
+}
+
+int pfdtd_get_response_shape(pfdtd_ctx *c, int *n_rec, int *n_steps) {
+  if (!c || !n_rec || !n_steps) return fail(PFDTD_E_ARG, "null argument");
+  *n_rec = c->n_rec;
+  *n_steps = c->resp_steps;
+  return PFDTD_OK;
+}
+
+int pfdtd_get_responses_f64(pfdtd_ctx *c, double *out, int64_t capacity) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const int64_t need = (int64_t)c->n_rec * c->resp_steps;
+  if (!out || capacity < need) return fail(PFDTD_E_ARG, "response buffer too small (%lld < %lld)", (long long)capacity, (long long)need);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (need) HIP_TRY(hipMemcpy(out, c->d_resp, sizeof(double) * need, hipMemcpyDeviceToHost));
+  return PFDTD_OK;
+}
+
+int pfdtd_get_responses_f32(pfdtd_ctx *c, float *out, int64_t capacity) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const int64_t need = (int64_t)c->n_rec * c->resp_steps;
+  if (!out || capacity < need) return fail(PFDTD_E_ARG, "response buffer too small (%lld < %lld)", (long long)capacity, (long long)need);
+  std::vector<double> tmp((size_t)need);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (need) HIP_TRY(hipMemcpy(tmp.data(), c->d_resp, sizeof(double) * need, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < need; ++i) out[i] = (float)tmp[(size_t)i];
+  return PFDTD_OK;
+}
+
+int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_out,
+                        uint8_t *pos_out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->d_P || c->precision < 0) return fail(PFDTD_E_STATE, "fields not prepared");
+  if (!p_out || !pos_out) return fail(PFDTD_E_ARG, "null output");
+  int64_t lim, U, V;
+  switch (orientation) {
+  case 0: lim = c->Z; U = c->X; V = c->Y; break;
+  case 1: lim = c->Y; U = c->X; V = c->Z; break;
+  case 2: lim = c->X; U = c->Y; V = c->Z; break;
+  default: return fail(PFDTD_E_ARG, "orientation %d not in {0,1,2}", orientation);
+  }
+  if (slice < 0 || slice >= lim)
+    return fail(PFDTD_E_ARG, "slice %lld out of bounds %lld, no capture made", (long long)slice, (long long)lim);
+  const int64_t n = U * V;
+  float *d_p = nullptr;
+  uint8_t *d_q = nullptr;
+  HIP_TRY(hipMalloc((void **)&d_p, sizeof(float) * n));
+  hipError_t e = hipMalloc((void **)&d_q, (size_t)n);
+  if (e == hipSuccess)
+    e = launch_capture_slice(c->precision, c->d_P, c->d_pos, (int)c->X, (int)c->Y, (int)c->Z,
+                             slice, orientation, c->scale_exp, d_p, d_q, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(p_out, d_p, sizeof(float) * n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(pos_out, d_q, (size_t)n, hipMemcpyDeviceToHost);
+  (void)hipFree(d_p);
+  if (d_q) (void)hipFree(d_q);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "capture_slice: %s", hipGetErrorString(e));
+  return PFDTD_OK;
+}
+
+int pfdtd_capture_field(pfdtd_ctx *c, float *out, int64_t capacity) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->d_P || c->precision < 0) return fail(PFDTD_E_STATE, "fields not prepared");
+  if (!out || capacity < c->N) return fail(PFDTD_E_ARG, "output buffer too small");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->precision == PFDTD_F32) {
+    HIP_TRY(hipMemcpy(out, c->d_P, sizeof(float) * c->N, hipMemcpyDeviceToHost));
+    return PFDTD_OK;
+  }
+  float *d_tmp = nullptr;
+  HIP_TRY(hipMalloc((void **)&d_tmp, sizeof(float) * c->N));
+  hipError_t e = launch_field_to_f32(c->precision, c->d_P, c->N, c->scale_exp, d_tmp, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, d_tmp, sizeof(float) * c->N, hipMemcpyDeviceToHost);
+  (void)hipFree(d_tmp);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "capture_field: %s", hipGetErrorString(e));
+  return PFDTD_OK;
+}
+
+int pfdtd_get_fields_raw(pfdtd_ctx *c, void *p_out, void *p_past_out, int64_t capacity_bytes) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->d_P) return fail(PFDTD_E_STATE, "fields not prepared");
+  if (capacity_bytes < (int64_t)c->field_bytes) return fail(PFDTD_E_ARG, "output buffer too small");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (p_out) HIP_TRY(hipMemcpy(p_out, c->d_P, c->field_bytes, hipMemcpyDeviceToHost));
+  if (p_past_out) HIP_TRY(hipMemcpy(p_past_out, c->d_Pp, c->field_bytes, hipMemcpyDeviceToHost));
+  return PFDTD_OK;
+}
+
+int pfdtd_set_kernel_timing(pfdtd_ctx *c, int enabled) {
+  if (!c) return fail(PFDTD_E_ARG, "null context");
+  c->kernel_timing = enabled != 0;
+  return PFDTD_OK;
+}
+
+int pfdtd_get_kernel_timing(pfdtd_ctx *c, double *update_ms_total, int64_t *update_launches) {
+  if (!c || !update_ms_total || !update_launches) return fail(PFDTD_E_ARG, "null argument");
+  *update_ms_total = c->upd_ms;
+  *update_launches = c->upd_launches;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_update_tiling(pfdtd_ctx *c, int rows, int zchunk) {
+  if (!c) return fail(PFDTD_E_ARG, "null context");
+  if (rows != 1 && rows != 2 && rows != 4)
+    return fail(PFDTD_E_ARG, "rows per thread %d not in {1,2,4}", rows);
+  if (zchunk < 1 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 1..65535", zchunk);
+  c->ry = rows;
+  c->zc = zchunk;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_poll_interval(pfdtd_ctx *c, int steps) {
+  if (!c) return fail(PFDTD_E_ARG, "null context");
+  if (steps < 1 || steps > 100000) return fail(PFDTD_E_ARG, "poll interval %d outside 1..100000", steps);
+  c->poll_interval = steps;
+  return PFDTD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// pfdtd_internal.h -- declarations shared by the HIP kernel and C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pfdtd {
+
+// Per-(K, material) update coefficients of the SRL_FORWARD rule
+// (src/kernels/kernels3d.cu:508-528):
+//   inv = 1/(1+beta_e), omb = 1-beta_e, tk = 2-K*lambda^2,
+//   beta_e = 0.5*(beta[m]*(6-K)*lambda).
+// Indexed [K*256 + material byte] for K in 0..6; row 7 is all zero and catches any K the
+// domain check should already have rejected.
+struct CoefF {
+  float inv, omb, tk, pad;
+};
+struct CoefD {
+  double inv, omb, tk, pad;
+};
+constexpr int kCoefRows = 8;
+constexpr int kCoefEntries = kCoefRows * 256;
+
+// Storage tags for the half-width field formats.
+struct h16 {
+  uint16_t bits;
+};
+struct b16 {
+  uint16_t bits;
+};
+
+// Result of the domain scan (validate_domain).
+struct DomainScan {
+  unsigned long long air;       // pos == 0x86
+  unsigned long long boundary;  // pos != 0 && pos != 0x86
+  unsigned int shell_inside;    // voxels on the outer layer with bit 7 set
+  unsigned int bad_k;           // inside voxels with K > 6
+  unsigned int mat_present[8];  // 256-bit set of material bytes that occur
+};
+
+// Update kernel launch geometry.
+struct UpdateGeom {
+  int X, Y, Z;
+  int ry;       // y rows per thread (1, 2 or 4)
+  int zc;       // interior slices per block
+  dim3 grid, block;
+};
+UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc);
+
+// Arguments of one update launch: fields in storage type S, coefficients of compute type C.
+// `air` holds the coefficients shared by every air voxel (position byte 0x86); air_uniform
+// is 0 when the table's K = 6 row differs between materials (non-finite admittances), in
+// which case every voxel reads the table.
+template <typename S, typename CT, typename C>
+struct UpdateArgs {
+  const uint8_t *pos;
+  const uint8_t *mat;
+  const S *P;
+  S *Pp;
+  const CT *coef;
+  CT air;
+  int air_uniform;
+  C lam2;
+};
+
+// Launchers (all asynchronous on `stream`; return the launch status).
+hipError_t launch_update_f32(const UpdateGeom &g, const UpdateArgs<float, CoefF, float> &a,
+                             hipStream_t stream);
+hipError_t launch_update_f64(const UpdateGeom &g, const UpdateArgs<double, CoefD, double> &a,
+                             hipStream_t stream);
+hipError_t launch_update_f16(const UpdateGeom &g, const UpdateArgs<h16, CoefF, float> &a,
+                             hipStream_t stream);
+hipError_t launch_update_bf16(const UpdateGeom &g, const UpdateArgs<b16, CoefF, float> &a,
+                              hipStream_t stream);
+
+// Receiver gather of time index `gather_step` (skipped when < 0) followed by injection of
+// source samples of time index `inject_step` (skipped when < 0) into the same field.
+struct InjectArgs {
+  const int64_t *rec_idx;  // -1: outside the grid
+  int n_rec;
+  double *resp;            // [n_rec][resp_steps]
+  int resp_steps;
+  int gather_step;
+  const int64_t *src_idx;  // -1: outside the grid
+  const int32_t *src_kind;
+  int n_src;
+  const float *sig_f;      // [n_src][n_samples]
+  const double *sig_d;
+  int n_samples;
+  int inject_step;
+  int soft_overwrite;
+  int scale_exp;
+};
+hipError_t launch_inject_gather(int precision, void *field, const InjectArgs &a,
+                                hipStream_t stream);
+
+hipError_t launch_classify_labels(const uint8_t *labels, uint8_t *pos, uint8_t *mat, int X,
+                                  int Y, int Z, hipStream_t stream);
+hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream);
+hipError_t launch_scan_domain(const uint8_t *pos, const uint8_t *mat, int X, int Y, int Z,
+                              DomainScan *d_out, hipStream_t stream);
+// Gathers a plane of the field (unscaled FP32) and of the position bytes.
+hipError_t launch_capture_slice(int precision, const void *field, const uint8_t *pos, int X,
+                                int Y, int Z, int64_t slice, int orientation, int scale_exp,
+                                float *out_p, uint8_t *out_pos, hipStream_t stream);
+// Whole field to unscaled FP32.
+hipError_t launch_field_to_f32(int precision, const void *field, int64_t n, int scale_exp,
+                               float *out, hipStream_t stream);
+
+}  // namespace pfdtd

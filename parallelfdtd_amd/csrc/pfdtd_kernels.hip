@@ -1,0 +1,519 @@
+// pfdtd_kernels.hip -- gfx950 kernels of the SRL_FORWARD time-stepping core.
+//
+// The pressure update is a bandwidth-bound 7-point stencil: per node it reads the position
+// and material bytes, P at the node and its six face neighbours, and P_past, and writes
+// P_next over P_past (src/kernels/kernels3d.cu:485-529). No MFMA is involved.
+//
+// Floating-point contraction is disabled in every kernel so each product and sum is a
+// separately rounded IEEE operation in the reference's operand order; with the per-(K, m)
+// coefficients built on the host in the same order, results are bit-identical to a CPU
+// evaluation of the reference expression.
+#include "pfdtd_internal.h"
+
+namespace pfdtd {
+
+// ---------------------------------------------------------------------------------------
+// Field storage formats
+// ---------------------------------------------------------------------------------------
+template <typename S>
+struct Fmt;
+
+template <>
+struct Fmt<float> {
+  typedef float C;
+  static __device__ __forceinline__ float ld(const float *p, int64_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(float *p, int64_t i, float v) { p[i] = v; }
+};
+
+template <>
+struct Fmt<double> {
+  typedef double C;
+  static __device__ __forceinline__ double ld(const double *p, int64_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(double *p, int64_t i, double v) { p[i] = v; }
+};
+
+template <>
+struct Fmt<h16> {
+  typedef float C;
+  static __device__ __forceinline__ float ld(const h16 *p, int64_t i) {
+    return (float)__builtin_bit_cast(_Float16, p[i].bits);
+  }
+  static __device__ __forceinline__ void st(h16 *p, int64_t i, float v) {
+    // The value must be rounded to FP32 first and then to binary16 (round to nearest even):
+    // without the register barrier the compiler folds a producing multiply into
+    // v_fma_mixlo_f16, which rounds the exact product once, straight to binary16.
+    asm volatile("" : "+v"(v));
+    p[i].bits = __builtin_bit_cast(uint16_t, (_Float16)v);
+  }
+};
+
+template <>
+struct Fmt<b16> {
+  typedef float C;
+  static __device__ __forceinline__ float ld(const b16 *p, int64_t i) {
+    return __uint_as_float((uint32_t)p[i].bits << 16);
+  }
+  static __device__ __forceinline__ void st(b16 *p, int64_t i, float v) {
+    uint32_t x = __float_as_uint(v);
+    if ((x & 0x7FFFFFFFu) > 0x7F800000u) {
+      p[i].bits = (uint16_t)((x >> 16) | 0x40u);  // quiet NaN
+      return;
+    }
+    x += 0x7FFFu + ((x >> 16) & 1u);  // round to nearest even
+    p[i].bits = (uint16_t)(x >> 16);
+  }
+};
+
+template <typename C>
+struct CoefOf;
+template <>
+struct CoefOf<float> {
+  typedef CoefF T;
+};
+template <>
+struct CoefOf<double> {
+  typedef CoefD T;
+};
+
+// ---------------------------------------------------------------------------------------
+// SRL_FORWARD update: row-segment tiles marching short z-chunks
+// ---------------------------------------------------------------------------------------
+// A block is 256 threads = four wavefronts side by side along x, so one block row is a
+// contiguous 1 KB (FP32) segment and every wavefront access is a full 256 B run. Each thread
+// owns RY consecutive y rows of its x column and marches them through `zc` slices, keeping
+// P[z-1] and P[z] of every row in registers (the z-neighbour queue); per slice only P[z+1],
+// P_past and the two byte arrays are new, the y-neighbours of inner rows come from registers
+// and the halo rows y0-1, y0+RY and the x-neighbours are cache hits of neighbouring threads'
+// loads. Short z-chunks keep the set of concurrently streamed slices small so the y-halo rows
+// and the slices above and below are still in L2 / Infinity Cache when they are re-read.
+// Air voxels (position byte 0x86) take their coefficients from kernel arguments; every other
+// voxel gathers its (K, material) entry from the table.
+constexpr int kTX = 256;
+
+template <typename S, int RY>
+__global__ __launch_bounds__(kTX) void srl_update_kernel(
+    const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
+    S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
+    typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
+    int X, int Y, int Z, int zc) {
+#pragma clang fp contract(off)
+  typedef typename Fmt<S>::C C;
+  const int x = blockIdx.x * kTX + threadIdx.x;
+  const int ybase = blockIdx.y * RY;
+  if (x >= X) return;
+  const int z0 = 1 + (int)blockIdx.z * zc;
+  const int z1 = min(z0 + zc, Z - 1);
+  const int64_t sX = X;
+  const int64_t XY = (int64_t)X * (int64_t)Y;
+  const bool xin = (x > 0) && (x < X - 1);
+
+  // Row r is live when ybase + r < Y; it is updated when additionally 0 < y < Y-1 and x is
+  // interior. Rows past the grid edge are clamped to Y-1 for loads and never stored.
+  bool live[RY], upd[RY];
+  int64_t row[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = ybase + r;
+    live[r] = y < Y;
+    upd[r] = live[r] && xin && y > 0 && y < Y - 1;
+    row[r] = (int64_t)min(y, Y - 1) * sX + x;
+  }
+  const int64_t halo_lo = (int64_t)max(ybase - 1, 0) * sX + x;
+  const int64_t halo_hi = (int64_t)min(ybase + RY, Y - 1) * sX + x;
+
+  C pm[RY], pc[RY];
+  int64_t base = (int64_t)z0 * XY;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    pm[r] = Fmt<S>::ld(P, base - XY + row[r]);
+    pc[r] = Fmt<S>::ld(P, base + row[r]);
+  }
+  for (int z = z0; z < z1; ++z, base += XY) {
+    C pn[RY], past[RY], xp[RY], xm[RY];
+    unsigned pb[RY], mb[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int64_t i = base + row[r];
+      pn[r] = Fmt<S>::ld(P, i + XY);
+      past[r] = Fmt<S>::ld(Pp, i);
+      pb[r] = pos[i];
+      mb[r] = mat[i];
+      xp[r] = Fmt<S>::ld(P, i + 1);
+      xm[r] = Fmt<S>::ld(P, i - 1);
+    }
+    const C ylo = Fmt<S>::ld(P, base + halo_lo);
+    const C yhi = Fmt<S>::ld(P, base + halo_hi);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const C yp = (r + 1 < RY) ? pc[r + 1] : yhi;
+      const C ym = (r > 0) ? pc[r - 1] : ylo;
+      // Sum order z+1, z-1, y+1, y-1, x+1, x-1 (kernels3d.cu:516-525).
+      C s = pn[r] + pm[r];
+      s = s + yp;
+      s = s + ym;
+      s = s + xp[r];
+      s = s + xm[r];
+      typename CoefOf<C>::T e = air;
+      if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * 256u + mb[r]];
+      const C a = (C)(pb[r] >> 7) * e.inv;
+      const C ret = a * ((e.tk * pc[r] + lam2 * s) - e.omb * past[r]);
+      if (live[r]) Fmt<S>::st(Pp, base + row[r], upd[r] ? ret : (C)0);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      pm[r] = pc[r];
+      pc[r] = pn[r];
+    }
+  }
+}
+
+UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc) {
+  UpdateGeom g;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  g.ry = ry;
+  const int zi = Z - 2;
+  g.zc = zc < 1 ? 1 : (zc > zi ? zi : zc);
+  const int bx = (X + kTX - 1) / kTX;
+  const int by = (Y + ry - 1) / ry;
+  const int nz = (zi + g.zc - 1) / g.zc;
+  g.grid = dim3(bx, by, nz);
+  g.block = dim3(kTX, 1, 1);
+  return g;
+}
+
+template <typename S, typename CT, typename C>
+static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> &a,
+                                hipStream_t stream) {
+  switch (g.ry) {
+#define PFDTD_RY_CASE(R)                                                                     \
+  case R:                                                                                    \
+    hipLaunchKernelGGL((srl_update_kernel<S, R>), g.grid, g.block, 0, stream, a.pos, a.mat,  \
+                       a.P, a.Pp, a.coef, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z, g.zc); \
+    break;
+    PFDTD_RY_CASE(1)
+    PFDTD_RY_CASE(2)
+    PFDTD_RY_CASE(4)
+#undef PFDTD_RY_CASE
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_update_f32(const UpdateGeom &g, const UpdateArgs<float, CoefF, float> &a,
+                             hipStream_t stream) {
+  return launch_update(g, a, stream);
+}
+hipError_t launch_update_f64(const UpdateGeom &g, const UpdateArgs<double, CoefD, double> &a,
+                             hipStream_t stream) {
+  return launch_update(g, a, stream);
+}
+hipError_t launch_update_f16(const UpdateGeom &g, const UpdateArgs<h16, CoefF, float> &a,
+                             hipStream_t stream) {
+  return launch_update(g, a, stream);
+}
+hipError_t launch_update_bf16(const UpdateGeom &g, const UpdateArgs<b16, CoefF, float> &a,
+                              hipStream_t stream) {
+  return launch_update(g, a, stream);
+}
+
+// ---------------------------------------------------------------------------------------
+// Receiver gather and source injection (one block)
+// ---------------------------------------------------------------------------------------
+// Gather reads the field just produced (response[step] = P^{step+1}, kernels3d.cu:164-173);
+// injection then applies the next step's samples to the same field, in source order, which
+// is where the reference's source loop of the next step writes them (kernels3d.cu:93-104).
+template <typename S>
+__global__ __launch_bounds__(256) void inject_gather_kernel(S *field, InjectArgs a) {
+#pragma clang fp contract(off)
+  typedef typename Fmt<S>::C C;
+  if (a.gather_step >= 0) {
+    const double inv_scale = ldexp(1.0, -a.scale_exp);
+    for (int r = threadIdx.x; r < a.n_rec; r += blockDim.x) {
+      const int64_t idx = a.rec_idx[r];
+      if (idx >= 0)
+        a.resp[(int64_t)r * a.resp_steps + a.gather_step] =
+            (double)Fmt<S>::ld(field, idx) * inv_scale;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && a.inject_step >= 0) {
+    const C scale = (C)ldexp(1.0, a.scale_exp);
+    for (int s = 0; s < a.n_src; ++s) {
+      const int64_t idx = a.src_idx[s];
+      if (idx < 0) continue;
+      C smp = (C)0;
+      if (a.inject_step < a.n_samples) {
+        const int64_t off = (int64_t)s * a.n_samples + a.inject_step;
+        if (sizeof(C) == sizeof(double))
+          smp = (C)a.sig_d[off];
+        else
+          smp = (C)a.sig_f[off];
+      }
+      smp = smp * scale;
+      if (a.src_kind[s] == 0 || a.soft_overwrite)
+        Fmt<S>::st(field, idx, smp);
+      else
+        Fmt<S>::st(field, idx, Fmt<S>::ld(field, idx) + smp);
+    }
+  }
+}
+
+hipError_t launch_inject_gather(int precision, void *field, const InjectArgs &a,
+                                hipStream_t stream) {
+  const dim3 grid(1), block(256);
+  switch (precision) {
+  case 0:
+    hipLaunchKernelGGL(inject_gather_kernel<float>, grid, block, 0, stream, (float *)field, a);
+    break;
+  case 1:
+    hipLaunchKernelGGL(inject_gather_kernel<double>, grid, block, 0, stream, (double *)field,
+                       a);
+    break;
+  case 2:
+    hipLaunchKernelGGL(inject_gather_kernel<h16>, grid, block, 0, stream, (h16 *)field, a);
+    break;
+  case 3:
+    hipLaunchKernelGGL(inject_gather_kernel<b16>, grid, block, 0, stream, (b16 *)field, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Domain construction and scan
+// ---------------------------------------------------------------------------------------
+// Label grid -> position/material bytes. Block = 256 consecutive x of one (y, z) row.
+__global__ __launch_bounds__(256) void classify_labels_kernel(const uint8_t *__restrict__ label,
+                                                              uint8_t *__restrict__ pos,
+                                                              uint8_t *__restrict__ mat, int X,
+                                                              int Y, int Z) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+  if (x >= X) return;
+  const int64_t XY = (int64_t)X * Y;
+  const int64_t i = (int64_t)z * XY + (int64_t)y * X + x;
+  if (label[i] != 0) {
+    pos[i] = 0;
+    mat[i] = 0;
+    return;
+  }
+  const bool inb[6] = {x > 0, x < X - 1, y > 0, y < Y - 1, z > 0, z < Z - 1};
+  const int64_t off[6] = {-1, 1, -(int64_t)X, (int64_t)X, -XY, XY};
+  int K = 0;
+  bool found = false;
+  unsigned m = 0;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const unsigned lab = inb[d] ? label[i + off[d]] : 1u;
+    if (lab == 0) {
+      ++K;
+    } else if (!found) {
+      found = true;
+      m = inb[d] ? lab - 1u : 0u;
+    }
+  }
+  pos[i] = (uint8_t)(0x80 | K);
+  mat[i] = (uint8_t)(K < 6 ? m : 0);
+}
+
+hipError_t launch_classify_labels(const uint8_t *labels, uint8_t *pos, uint8_t *mat, int X,
+                                  int Y, int Z, hipStream_t stream) {
+  const dim3 grid((X + 255) / 256, Y, Z), block(256);
+  hipLaunchKernelGGL(classify_labels_kernel, grid, block, 0, stream, labels, pos, mat, X, Y, Z);
+  return hipGetLastError();
+}
+
+// toBilbao (src/kernels/cudaMesh.cu:328-361) applied in place to voxelizer codes.
+__global__ __launch_bounds__(256) void codes_to_bilbao_kernel(uint8_t *pos, uint8_t *mat,
+                                                              int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const unsigned k = pos[i];
+    if (k == 0) {
+      pos[i] = 0;
+      mat[i] = 0;
+    } else if (k <= 8) {
+      pos[i] = 0x83;
+    } else if (k <= 20) {
+      pos[i] = 0x84;
+    } else if (k <= 26) {
+      pos[i] = 0x85;
+    } else if (k == 27) {
+      pos[i] = 0x86;
+    }
+  }
+}
+
+hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream) {
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(codes_to_bilbao_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, pos,
+                     mat, n);
+  return hipGetLastError();
+}
+
+// Counts air/boundary voxels (calcBoundaries, cudaMesh.cu:500-514) and checks the
+// preconditions the update relies on. One block per (y, z) row segment of 256 x.
+__global__ __launch_bounds__(256) void scan_domain_kernel(const uint8_t *__restrict__ pos,
+                                                          const uint8_t *__restrict__ mat, int X,
+                                                          int Y, int Z, DomainScan *out) {
+  __shared__ unsigned int s_air, s_bnd, s_shell, s_badk;
+  __shared__ unsigned int s_mat[8];
+  if (threadIdx.x < 8) s_mat[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    s_air = 0;
+    s_bnd = 0;
+    s_shell = 0;
+    s_badk = 0;
+  }
+  __syncthreads();
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+  if (x < X) {
+    const int64_t i = ((int64_t)z * Y + y) * X + x;
+    const unsigned p = pos[i];
+    const unsigned m = mat[i];
+    if (p == 0x86u) atomicAdd(&s_air, 1u);
+    if (p != 0u && p != 0x86u) atomicAdd(&s_bnd, 1u);
+    const bool shell = (x == 0 || x == X - 1 || y == 0 || y == Y - 1 || z == 0 || z == Z - 1);
+    if (shell && (p >> 7)) atomicAdd(&s_shell, 1u);
+    if ((p >> 7) && (p & 0x7Fu) > 6u) atomicAdd(&s_badk, 1u);
+    atomicOr(&s_mat[m >> 5], 1u << (m & 31u));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_air) atomicAdd(&out->air, (unsigned long long)s_air);
+    if (s_bnd) atomicAdd(&out->boundary, (unsigned long long)s_bnd);
+    if (s_shell) atomicAdd(&out->shell_inside, s_shell);
+    if (s_badk) atomicAdd(&out->bad_k, s_badk);
+  }
+  if (threadIdx.x < 8 && s_mat[threadIdx.x]) atomicOr(&out->mat_present[threadIdx.x], s_mat[threadIdx.x]);
+}
+
+hipError_t launch_scan_domain(const uint8_t *pos, const uint8_t *mat, int X, int Y, int Z,
+                              DomainScan *d_out, hipStream_t stream) {
+  const dim3 grid((X + 255) / 256, Y, Z), block(256);
+  hipLaunchKernelGGL(scan_domain_kernel, grid, block, 0, stream, pos, mat, X, Y, Z, d_out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Capture
+// ---------------------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(256) void capture_slice_kernel(const S *__restrict__ field,
+                                                            const uint8_t *__restrict__ pos,
+                                                            int X, int Y, int Z, int64_t slice,
+                                                            int orientation, float inv_scale,
+                                                            float *out_p, uint8_t *out_pos) {
+  // Plane coordinates (u fastest, v): xy -> (x, y), xz -> (x, z), yz -> (y, z).
+  const int U = orientation == 2 ? Y : X;
+  const int V = orientation == 0 ? Y : Z;
+  const int64_t n = (int64_t)U * V;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * 256) {
+    const int u = (int)(t % U);
+    const int v = (int)(t / U);
+    int64_t x, y, z;
+    if (orientation == 0) {
+      x = u;
+      y = v;
+      z = slice;
+    } else if (orientation == 1) {
+      x = u;
+      y = slice;
+      z = v;
+    } else {
+      x = slice;
+      y = u;
+      z = v;
+    }
+    const int64_t i = (z * Y + y) * X + x;
+    out_p[t] = (float)Fmt<S>::ld(field, i) * inv_scale;
+    out_pos[t] = pos[i];
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void field_to_f32_kernel(const S *__restrict__ field,
+                                                           int64_t n, float inv_scale,
+                                                           float *out) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * 256)
+    out[t] = (float)Fmt<S>::ld(field, t) * inv_scale;
+}
+
+static unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_capture_slice(int precision, const void *field, const uint8_t *pos, int X,
+                                int Y, int Z, int64_t slice, int orientation, int scale_exp,
+                                float *out_p, uint8_t *out_pos, hipStream_t stream) {
+  const int U = orientation == 2 ? Y : X;
+  const int V = orientation == 0 ? Y : Z;
+  const dim3 grid(grid_for((int64_t)U * V)), block(256);
+  const float inv_scale = ldexpf(1.f, -scale_exp);
+  switch (precision) {
+  case 0:
+    hipLaunchKernelGGL(capture_slice_kernel<float>, grid, block, 0, stream, (const float *)field,
+                       pos, X, Y, Z, slice, orientation, inv_scale, out_p, out_pos);
+    break;
+  case 1:
+    hipLaunchKernelGGL(capture_slice_kernel<double>, grid, block, 0, stream,
+                       (const double *)field, pos, X, Y, Z, slice, orientation, inv_scale, out_p,
+                       out_pos);
+    break;
+  case 2:
+    hipLaunchKernelGGL(capture_slice_kernel<h16>, grid, block, 0, stream, (const h16 *)field,
+                       pos, X, Y, Z, slice, orientation, inv_scale, out_p, out_pos);
+    break;
+  case 3:
+    hipLaunchKernelGGL(capture_slice_kernel<b16>, grid, block, 0, stream, (const b16 *)field,
+                       pos, X, Y, Z, slice, orientation, inv_scale, out_p, out_pos);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_field_to_f32(int precision, const void *field, int64_t n, int scale_exp,
+                               float *out, hipStream_t stream) {
+  const dim3 grid(grid_for(n)), block(256);
+  const float inv_scale = ldexpf(1.f, -scale_exp);
+  switch (precision) {
+  case 0:
+    hipLaunchKernelGGL(field_to_f32_kernel<float>, grid, block, 0, stream, (const float *)field,
+                       n, inv_scale, out);
+    break;
+  case 1:
+    hipLaunchKernelGGL(field_to_f32_kernel<double>, grid, block, 0, stream,
+                       (const double *)field, n, inv_scale, out);
+    break;
+  case 2:
+    hipLaunchKernelGGL(field_to_f32_kernel<h16>, grid, block, 0, stream, (const h16 *)field, n,
+                       inv_scale, out);
+    break;
+  case 3:
+    hipLaunchKernelGGL(field_to_f32_kernel<b16>, grid, block, 0, stream, (const b16 *)field, n,
+                       inv_scale, out);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pfdtd
