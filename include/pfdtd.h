@@ -159,13 +159,23 @@ int pfdtd_capture_field(pfdtd_ctx *ctx, float *out, int64_t capacity);
 /* Whole current and past fields in storage precision (raw bytes), for checks. */
 int pfdtd_get_fields_raw(pfdtd_ctx *ctx, void *p_out, void *p_past_out, int64_t capacity_bytes);
 
-/* Timing of the update kernel inside pfdtd_run: when enabled, every update launch is bracketed
- * by HIP events on the context's stream and the summed device time is reported. */
+/* Timing of the update kernels inside pfdtd_run: when enabled, every update launch is bracketed
+ * by HIP events on the context's stream. update_ms_total is their summed device time and
+ * update_launches the number of time steps they advanced (a two-level launch counts 2). */
 int pfdtd_set_kernel_timing(pfdtd_ctx *ctx, int enabled);
 /* Steps between host polls of the interrupt/progress callbacks inside pfdtd_run (default 100,
  * PROGRESS_MOD in src/kernels/kernels3d.h:36). The reference polls the interrupt every step
  * (kernels3d.cu:86); polling here costs one stream synchronisation per interval. */
 int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
+/* Time levels advanced per sweep of the field (1 or 2; 0 = default: 1 for FP32, 2 for FP64 and
+ * the 2-byte storage formats, the faster path for each on MI355X) and slices per z-chunk of the
+ * two-level update (default 32). With 2, pfdtd_run advances pairs of steps in one kernel that
+ * reads P^n, P^{n-1} and the byte planes once and writes P^{n+1}, P^{n+2} (four field buffers;
+ * sources of step n+1 injected and receivers of step n gathered inside the kernel), falling back
+ * to single steps for an odd remainder and whenever a source sits on slice z = 0 or z = Z-1.
+ * Results are bit-identical to single steps. No reference counterpart: the reference's loop
+ * (kernels3d.cu:83-191) launches one update per step. */
+int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
 /* Update-kernel tiling: y rows per thread (1, 2 or 4; default 2) and slices per z-chunk
  * (default 8). Results do not depend on it (every voxel's arithmetic is the same); it only
  * changes the streaming pattern. Replaces the fixed 32x4 block of App::initializeMesh

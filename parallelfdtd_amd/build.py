@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_SOURCES = ["pfdtd_kernels.hip", "pfdtd_capi.hip"]
+LIB_SOURCES = ["pfdtd_kernels.hip", "pfdtd_fused.hip", "pfdtd_capi.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

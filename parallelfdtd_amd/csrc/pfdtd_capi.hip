@@ -89,23 +89,28 @@ struct pfdtd_ctx {
   int air_uniform = 0;    // 1 when every material gives the same K = 6 entry (both tables)
   int ry = 2;             // y rows per thread of the update kernel
   int zc = 8;             // slices per z-chunk of the update kernel
+  int levels = 0;         // time levels per sweep: 1, 2, or 0 = by precision
+  int fzc = 32;           // slices per z-chunk of the two-level update
 
   // sources / receivers
   int n_src = 0, n_samples = 0;
   std::vector<int32_t> src_xyz, src_kind;
   int64_t *d_src_idx = nullptr;
+  int32_t *d_src_xyz = nullptr;  // grid coordinates, (-2^30,-2^30,-2^30) outside the grid
   int32_t *d_src_kind = nullptr;
   float *d_sig_f = nullptr;
   double *d_sig_d = nullptr;
   int n_rec = 0;
   std::vector<int32_t> rec_xyz;
   int64_t *d_rec_idx = nullptr;
+  int32_t *d_rec_xyz = nullptr;
 
   // fields
   int precision = -1;
   int scale_exp = 0;      // scale of the allocated fields
   int scale_req = 0;      // requested F16/BF16 storage scale (pfdtd_prepare_fields)
-  void *d_P = nullptr, *d_Pp = nullptr;
+  void *d_P = nullptr, *d_Pp = nullptr;    // current and past field
+  void *d_P2 = nullptr, *d_Pp2 = nullptr;  // spare pair written by the two-level update
   size_t field_bytes = 0;
 
   // responses
@@ -139,6 +144,20 @@ int64_t lin_idx(const pfdtd_ctx *c, const int32_t *q) {
   if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= c->X || q[1] >= c->Y || q[2] >= c->Z)
     return -1;
   return ((int64_t)q[2] * c->Y + q[1]) * c->X + q[0];
+}
+
+constexpr int32_t kOutside = -(1 << 30);
+
+// Device copy of grid coordinates with entries outside the grid replaced by kOutside.
+int upload_xyz(const pfdtd_ctx *c, const std::vector<int32_t> &xyz, int32_t *dst) {
+  const size_t n = xyz.size() / 3;
+  if (!n) return PFDTD_OK;
+  std::vector<int32_t> v(xyz);
+  for (size_t i = 0; i < n; ++i)
+    if (!c->has_domain || lin_idx(c, &v[3 * i]) < 0) v[3 * i] = v[3 * i + 1] = v[3 * i + 2] = kOutside;
+  hipError_t e = hipMemcpy(dst, v.data(), sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "coordinate upload: %s", hipGetErrorString(e));
+  return PFDTD_OK;
 }
 
 // kernels3d.cu:513 (literal `mat*20*+params[3]`) vs kernels3d.cu:639 (intended)
@@ -192,6 +211,23 @@ int scan_domain(pfdtd_ctx *c) {
   return PFDTD_OK;
 }
 
+// Time levels per sweep used for a precision: the explicit setting, else the measured best on
+// MI355X (one level for FP32, whose single-level kernel already runs at the read+write
+// bandwidth ceiling; two for FP64 and the 2-byte storage formats).
+int levels_for(const pfdtd_ctx *c, int precision) {
+  if (c->levels) return c->levels;
+  return precision == PFDTD_F32 ? 1 : 2;
+}
+
+void free_fields(pfdtd_ctx *c) {
+  dfree(c->d_P);
+  dfree(c->d_Pp);
+  dfree(c->d_P2);
+  dfree(c->d_Pp2);
+  c->field_bytes = 0;
+  c->precision = -1;
+}
+
 int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
   if (X < 3 || Y < 3 || Z < 3)
     return fail(PFDTD_E_ARG, "grid %lldx%lldx%lld: every dimension must be >= 3",
@@ -206,10 +242,7 @@ int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
     c->dom_cap = 0;
   }
   // fields sized for the previous grid are dropped
-  dfree(c->d_P);
-  dfree(c->d_Pp);
-  c->precision = -1;
-  c->field_bytes = 0;
+  free_fields(c);
   if (!c->d_pos) {
     hipError_t e1 = hipMalloc((void **)&c->d_pos, (size_t)N);
     hipError_t e2 = e1 == hipSuccess ? hipMalloc((void **)&c->d_mat, (size_t)N) : e1;
@@ -239,12 +272,16 @@ int finish_domain(pfdtd_ctx *c) {
     for (int s = 0; s < c->n_src; ++s) idx[s] = lin_idx(c, &c->src_xyz[3 * s]);
     HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src,
                       hipMemcpyHostToDevice));
+    rc = upload_xyz(c, c->src_xyz, c->d_src_xyz);
+    if (rc) return rc;
   }
   if (c->n_rec) {
     std::vector<int64_t> idx(c->n_rec);
     for (int r = 0; r < c->n_rec; ++r) idx[r] = lin_idx(c, &c->rec_xyz[3 * r]);
     HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * c->n_rec,
                       hipMemcpyHostToDevice));
+    rc = upload_xyz(c, c->rec_xyz, c->d_rec_xyz);
+    if (rc) return rc;
   }
   return PFDTD_OK;
 }
@@ -279,22 +316,22 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
     return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
   if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
   const size_t bytes = es * (size_t)c->N;
-  if (c->d_P && c->field_bytes == bytes) {
+  const bool want4 = levels_for(c, precision) == 2;
+  if (c->d_P && c->field_bytes == bytes && (!want4 || c->d_P2)) {
     c->precision = precision;
     c->scale_exp = scale_exp;
     return PFDTD_OK;
   }
-  dfree(c->d_P);
-  dfree(c->d_Pp);
-  hipError_t e1 = hipMalloc(&c->d_P, bytes);
-  hipError_t e2 = e1 == hipSuccess ? hipMalloc(&c->d_Pp, bytes) : e1;
-  if (e1 != hipSuccess || e2 != hipSuccess) {
-    dfree(c->d_P);
-    dfree(c->d_Pp);
+  free_fields(c);
+  hipError_t e = hipMalloc(&c->d_P, bytes);
+  if (e == hipSuccess) e = hipMalloc(&c->d_Pp, bytes);
+  if (e == hipSuccess && want4) e = hipMalloc(&c->d_P2, bytes);
+  if (e == hipSuccess && want4) e = hipMalloc(&c->d_Pp2, bytes);
+  if (e != hipSuccess) {
+    free_fields(c);
     (void)hipGetLastError();
-    c->field_bytes = 0;
-    c->precision = -1;
-    return fail(PFDTD_E_NOMEM, "cannot allocate two %zu-byte pressure fields", bytes);
+    return fail(PFDTD_E_NOMEM, "cannot allocate %d pressure fields of %zu bytes", want4 ? 4 : 2,
+                bytes);
   }
   c->field_bytes = bytes;
   c->precision = precision;
@@ -305,6 +342,8 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
 int zero_fields(pfdtd_ctx *c) {
   HIP_TRY(hipMemsetAsync(c->d_P, 0, c->field_bytes, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_Pp, 0, c->field_bytes, c->stream));
+  if (c->d_P2) HIP_TRY(hipMemsetAsync(c->d_P2, 0, c->field_bytes, c->stream));
+  if (c->d_Pp2) HIP_TRY(hipMemsetAsync(c->d_Pp2, 0, c->field_bytes, c->stream));
   c->last_direction = 1;
   return PFDTD_OK;
 }
@@ -362,6 +401,67 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
     return launch_update_bf16(g, UpdateArgs<b16, CoefF, float>{c->d_pos, c->d_mat,
                               (const b16 *)c->d_P, (b16 *)c->d_Pp, c->d_coef_f, c->air_f,
                               c->air_uniform, c->lam2_f}, c->stream);
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
+// The two-level update needs the spare pair and no source on the never-updated slices
+// z = 0 and z = Z-1 (their values would have to be carried between the four buffers).
+bool fused_eligible(const pfdtd_ctx *c) {
+  if (levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2) return false;
+  for (int s = 0; s < c->n_src; ++s) {
+    const int32_t *q = &c->src_xyz[3 * s];
+    if (lin_idx(c, q) >= 0 && (q[2] == 0 || q[2] == c->Z - 1)) return false;
+  }
+  return true;
+}
+
+template <typename S, typename CT, typename C>
+Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step) {
+  Fused2Args<S, CT, C> a;
+  a.pos = c->d_pos;
+  a.mat = c->d_mat;
+  a.A = (const S *)c->d_P;
+  a.B = (const S *)c->d_Pp;
+  a.Cn = (S *)c->d_Pp2;
+  a.Dn = (S *)c->d_P2;
+  a.coef = coef;
+  a.air = air;
+  a.air_uniform = c->air_uniform;
+  a.lam2 = lam2;
+  a.src_xyz = c->d_src_xyz;
+  a.src_kind = c->d_src_kind;
+  a.n_src = c->n_src;
+  a.sig_f = c->d_sig_f;
+  a.sig_d = c->d_sig_d;
+  a.n_samples = c->n_samples;
+  a.inject_step = c->n_src > 0 ? step + 1 : -1;
+  a.soft_overwrite = (c->flags & PFDTD_SOFT_OVERWRITE) ? 1 : 0;
+  a.scale_exp = c->scale_exp;
+  a.rec_xyz = c->d_rec_xyz;
+  a.n_rec = c->n_rec;
+  a.resp = c->d_resp;
+  a.resp_steps = c->resp_steps;
+  a.gather_step = step;
+  return a;
+}
+
+// Steps `step` and `step+1` in one sweep: P^{step+1} lands in d_Pp2, P^{step+2} in d_P2.
+hipError_t launch_update2_any(pfdtd_ctx *c, const FusedGeom &g, int step) {
+  switch (c->precision) {
+  case PFDTD_F32:
+    return launch_update2_f32(g, fused_args<float>(c, c->d_coef_f, c->air_f, c->lam2_f, step),
+                              c->stream);
+  case PFDTD_F64:
+    return launch_update2_f64(g, fused_args<double>(c, c->d_coef_d, c->air_d, c->lam2_d, step),
+                              c->stream);
+  case PFDTD_F16:
+    return launch_update2_f16(g, fused_args<h16>(c, c->d_coef_f, c->air_f, c->lam2_f, step),
+                              c->stream);
+  case PFDTD_BF16:
+    return launch_update2_bf16(g, fused_args<b16>(c, c->d_coef_f, c->air_f, c->lam2_f, step),
+                               c->stream);
   default:
     return hipErrorInvalidValue;
   }
@@ -434,12 +534,13 @@ int pfdtd_ctx_destroy(pfdtd_ctx *c) {
   dfree(c->d_coef_f);
   dfree(c->d_coef_d);
   dfree(c->d_src_idx);
+  dfree(c->d_src_xyz);
   dfree(c->d_src_kind);
   dfree(c->d_sig_f);
   dfree(c->d_sig_d);
   dfree(c->d_rec_idx);
-  dfree(c->d_P);
-  dfree(c->d_Pp);
+  dfree(c->d_rec_xyz);
+  free_fields(c);
   dfree(c->d_resp);
   for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->ev_begin) (void)hipEventDestroy(c->ev_begin);
@@ -573,6 +674,7 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
       return fail(PFDTD_E_ARG, "source %d: unknown kind %d", s, kind[s]);
   HIP_TRY(hipStreamSynchronize(c->stream));
   dfree(c->d_src_idx);
+  dfree(c->d_src_xyz);
   dfree(c->d_src_kind);
   dfree(c->d_sig_f);
   dfree(c->d_sig_d);
@@ -593,6 +695,11 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
     for (int s = 0; s < n_src; ++s) idx[s] = lin_idx(c, &c->src_xyz[3 * s]);
   HIP_TRY(hipMalloc((void **)&c->d_src_idx, sizeof(int64_t) * n_src));
   HIP_TRY(hipMalloc((void **)&c->d_src_kind, sizeof(int32_t) * n_src));
+  HIP_TRY(hipMalloc((void **)&c->d_src_xyz, sizeof(int32_t) * 3 * n_src));
+  {
+    int rc2 = upload_xyz(c, c->src_xyz, c->d_src_xyz);
+    if (rc2) return rc2;
+  }
   HIP_TRY(hipMalloc((void **)&c->d_sig_f, sizeof(float) * (ns ? ns : 1)));
   HIP_TRY(hipMalloc((void **)&c->d_sig_d, sizeof(double) * (ns ? ns : 1)));
   HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * n_src, hipMemcpyHostToDevice));
@@ -613,6 +720,7 @@ int pfdtd_set_receivers(pfdtd_ctx *c, int n_rec, const int32_t *xyz) {
   if (n_rec > 0 && !xyz) return fail(PFDTD_E_ARG, "null receiver array");
   HIP_TRY(hipStreamSynchronize(c->stream));
   dfree(c->d_rec_idx);
+  dfree(c->d_rec_xyz);
   c->n_rec = 0;
   c->rec_xyz.assign(xyz, xyz + 3 * (size_t)n_rec);
   c->resp_steps = 0;
@@ -621,6 +729,11 @@ int pfdtd_set_receivers(pfdtd_ctx *c, int n_rec, const int32_t *xyz) {
   if (c->has_domain)
     for (int r = 0; r < n_rec; ++r) idx[r] = lin_idx(c, &c->rec_xyz[3 * r]);
   HIP_TRY(hipMalloc((void **)&c->d_rec_idx, sizeof(int64_t) * n_rec));
+  HIP_TRY(hipMalloc((void **)&c->d_rec_xyz, sizeof(int32_t) * 3 * n_rec));
+  {
+    int rc2 = upload_xyz(c, c->rec_xyz, c->d_rec_xyz);
+    if (rc2) return rc2;
+  }
   HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * n_rec, hipMemcpyHostToDevice));
   c->n_rec = n_rec;
   return PFDTD_OK;
@@ -671,30 +784,49 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   // step 0's source samples go into the (zero) current field before the first update
   if (n_steps > 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, 0), c->stream));
 
+  const bool fused = fused_eligible(c);
+  const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision);
   int done = 0;
   auto wall0 = std::chrono::steady_clock::now();
   while (done < n_steps) {
     if (interrupt && interrupt(user)) break;  // kernels3d.cu:86-89
     const int nb = std::min(bundle, n_steps - done);
-    for (int k = 0; k < nb; ++k) {
+    int nev = 0;
+    for (int k = 0; k < nb;) {
       const int step = done + k;
-      if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * k], c->stream));
-      HIP_TRY(launch_update_any(c, g));
-      if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * k + 1], c->stream));
-      // after the update the new field lives in d_Pp: gather this step, inject the next
-      const int next = step + 1 < n_steps ? step + 1 : -1;
-      HIP_TRY(launch_inject_gather(c->precision, c->d_Pp, inject_args(c, step, next), c->stream));
-      std::swap(c->d_P, c->d_Pp);  // flipPressurePointers, cudaMesh.h:766-779
+      if (fused && k + 2 <= nb) {
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev], c->stream));
+        HIP_TRY(launch_update2_any(c, fg, step));
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev + 1], c->stream));
+        ++nev;
+        // P^{step+2} (d_P2): gather step+1, inject step+2
+        const int next = step + 2 < n_steps ? step + 2 : -1;
+        HIP_TRY(launch_inject_gather(c->precision, c->d_P2, inject_args(c, step + 1, next), c->stream));
+        std::swap(c->d_P, c->d_P2);
+        std::swap(c->d_Pp, c->d_Pp2);
+        k += 2;
+        c->upd_launches += 2;
+      } else {
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev], c->stream));
+        HIP_TRY(launch_update_any(c, g));
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev + 1], c->stream));
+        ++nev;
+        // after the update the new field lives in d_Pp: gather this step, inject the next
+        const int next = step + 1 < n_steps ? step + 1 : -1;
+        HIP_TRY(launch_inject_gather(c->precision, c->d_Pp, inject_args(c, step, next), c->stream));
+        std::swap(c->d_P, c->d_Pp);  // flipPressurePointers, cudaMesh.h:766-779
+        k += 1;
+        c->upd_launches += 1;
+      }
     }
     if (interrupt || progress || c->kernel_timing) {
       HIP_TRY(hipStreamSynchronize(c->stream));
       if (c->kernel_timing) {
-        for (int k = 0; k < nb; ++k) {
+        for (int k = 0; k < nev; ++k) {
           float ms = 0.f;
           HIP_TRY(hipEventElapsedTime(&ms, c->events[2 * k], c->events[2 * k + 1]));
           c->upd_ms += ms;
         }
-        c->upd_launches += nb;
       }
     }
     const int first = done;
@@ -863,6 +995,20 @@ int pfdtd_set_update_tiling(pfdtd_ctx *c, int rows, int zchunk) {
   if (zchunk < 1 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 1..65535", zchunk);
   c->ry = rows;
   c->zc = zchunk;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_time_levels(pfdtd_ctx *c, int levels, int zchunk) {
+  if (!c) return fail(PFDTD_E_ARG, "null context");
+  if (levels < 0 || levels > 2) return fail(PFDTD_E_ARG, "time levels %d not in {0,1,2}", levels);
+  if (zchunk < 1 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 1..65535", zchunk);
+  if (levels != c->levels) {
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_fields(c);  // reallocated with the right number of buffers at the next run
+  }
+  c->levels = levels;
+  c->fzc = zchunk;
   return PFDTD_OK;
 }
 

@@ -1,0 +1,352 @@
+// pfdtd_fused.hip -- two SRL_FORWARD time steps per sweep (temporal blocking).
+//
+// Reads P^n (A), P^{n-1} (B) and the byte planes once and writes P^{n+1} (C) and P^{n+2} (D):
+// 18 bytes of HBM traffic per two FP32 node-updates instead of 28. The four field buffers are
+// distinct, so blocks never read what another block writes and need no synchronisation.
+//
+// Tiling: each wavefront owns a 60-column x 8-row x-y tile and one z-chunk. Its 64 lanes cover
+// the 60 owned columns plus two halo columns on each side. P^{n+1} is computed on the owned tile
+// plus a one-voxel ring (columns lanes 1..62, 10 rows), from P^n on a two-voxel ring (12 rows);
+// P^{n+2} is then computed on the owned tile. x-neighbours come from cross-lane shuffles, y- and
+// z-neighbours from registers: the wave marches z with P^n of three slices and P^{n+1} of three
+// slices in registers, the second time level trailing the first by one slice.
+//
+// Source injection of step n+1 and the receiver gather of step n happen inside the kernel on
+// P^{n+1}, in the reference's order (gather before the next injection; sources in index order),
+// so every node's arithmetic -- and storage rounding -- equals two single steps bit for bit.
+#include "pfdtd_device.h"
+
+#include <type_traits>
+
+namespace pfdtd {
+
+constexpr int kFO = 60;       // owned columns per wave tile
+constexpr int kFWaves = 4;    // wave tiles per block, stacked in y
+constexpr int kFListCap = 32; // per-wave list capacity for sources / receivers (LDS)
+
+__device__ __forceinline__ bool in_box(const int32_t *q, int x0, int x1, int y0, int y1, int z0,
+                                       int z1) {
+  return q[0] >= x0 && q[0] < x1 && q[1] >= y0 && q[1] < y1 && q[2] >= z0 && q[2] < z1;
+}
+
+// Order-preserving wave-level compaction of the entries of `xyz` inside a box into `list`.
+// Returns the count, or -1 when it exceeds the capacity (the caller then scans all entries).
+__device__ __forceinline__ int wave_collect(const int32_t *xyz, int n, int x0, int x1, int y0,
+                                            int y1, int z0, int z1, int *list, int lane) {
+  int cnt = 0;
+  for (int base = 0; base < n; base += 64) {
+    const int s = base + lane;
+    const bool hit = s < n && in_box(xyz + 3 * s, x0, x1, y0, y1, z0, z1);
+    const uint64_t m = __ballot(hit);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (hit && cnt >= 0 && cnt + pre < kFListCap) list[cnt + pre] = s;
+    if (cnt >= 0) cnt += __popcll(m);
+    if (cnt > kFListCap) cnt = -1;
+  }
+  return cnt;
+}
+
+// x-neighbours across the wavefront with DPP row shifts: wave_shl:1 gives lane i the value of
+// lane i+1, wave_shr:1 the value of lane i-1 (lanes without a source get 0; they are halo
+// lanes whose results are never used).
+__device__ __forceinline__ float lane_next(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_next(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_prev(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Packed position|material<<8 of rows 2i and 2i+1 in one register.
+template <int NB>
+__device__ __forceinline__ unsigned pm_of(const uint32_t (&b)[NB], int j) {
+  return (b[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+}
+
+// Owned rows per wave tile: 8 for 4- and 2-byte fields, 4 for FP64 (register budget).
+template <typename S>
+struct FusedTile {
+  static constexpr int TY = 8;
+};
+template <>
+struct FusedTile<double> {
+  static constexpr int TY = 4;
+};
+
+template <typename S, typename CT, typename C>
+__global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S, CT, C> a, int X, int Y, int Z, int zc) {
+#pragma clang fp contract(off)
+  constexpr int kFTY = FusedTile<S>::TY;
+  constexpr int NA = kFTY + 4;   // P^n rows per wave tile (2-voxel y halo)
+  constexpr int NQ = kFTY + 2;   // P^{n+1} rows (1-voxel y halo)
+  constexpr int NB = NQ / 2;     // packed byte registers
+  __shared__ int s_src[kFWaves][kFListCap];
+  __shared__ int s_rec[kFWaves][kFListCap];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int xb = blockIdx.x * kFO;  // first owned column
+  const int x = xb - 2 + lane;      // this lane's column
+  const int yt = (blockIdx.y * kFWaves + wave) * kFTY;  // first owned row
+  const int z0 = 1 + (int)blockIdx.z * zc;
+  const int z1 = min(z0 + zc, Z - 1);  // owned slices [z0, z1)
+
+  // Sources that land on this wave's P^{n+1} region (owned tile + 1-voxel ring, slices
+  // z0-1 .. z1) and receivers on its owned tile.
+  const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kFO + 1, yt - 1, yt + kFTY + 1,
+                                z0 - 1, z1 + 1, s_src[wave], lane);
+  const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kFO, yt, yt + kFTY, z0, z1,
+                                s_rec[wave], lane);
+  __syncthreads();
+  if (yt >= Y) return;  // no barrier follows
+
+  const size_t XY = (size_t)X * (size_t)Y;
+  const int xc = min(max(x, 0), X - 1);
+  // Rows r = 0..11 of the P^n block are y = yt - 2 + r (clamped to the grid for loads).
+  uint32_t ro[NA];
+  uint32_t imask = 0;  // bit r: node (x, y_r) is updated (not on the outer layer)
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    const int y = yt - 2 + r;
+    ro[r] = (uint32_t)(min(max(y, 0), Y - 1) * X + xc);
+    if (x > 0 && x < X - 1 && y > 0 && y < Y - 1) imask |= 1u << r;
+  }
+  const bool owned_lane = lane >= 2 && lane < 2 + kFO && x < X;
+  uint32_t smask = 0;  // bit o: owned row o (y = yt + o) is stored by this lane
+#pragma unroll
+  for (int o = 0; o < kFTY; ++o)
+    if (owned_lane && yt + o < Y) smask |= 1u << o;
+  const double inv_scale = ldexp(1.0, -a.scale_exp);
+  const C scale = (C)ldexp(1.0, a.scale_exp);
+
+  const typename CoefOf<C>::T *__restrict__ coef = a.coef;
+  const typename CoefOf<C>::T air = a.air;
+  const C lam2 = a.lam2;
+  // TABLE: gather the (K, material) entry; otherwise every voxel of the stage is air.
+  auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
+    const unsigned pb = pm & 0xFFu;
+    typename CoefOf<C>::T e = air;
+    if (decltype(table)::value) e = coef[min(pb & 0x7Fu, 7u) * 256u + (pm >> 8)];
+    const C aa = (C)(pb >> 7) * e.inv;
+    return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
+  };
+  const bool air_uniform = a.air_uniform != 0;
+  // wave-uniform: does any of the ten packed rows hold a voxel other than air (0x86)?
+  auto any_nonair = [&](const uint32_t (&b)[NB]) -> bool {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) m |= ((b[i] & 0xFFu) ^ 0x86u) | (((b[i] >> 16) & 0xFFu) ^ 0x86u);
+    return !air_uniform || __ballot(m != 0u) != 0;
+  };
+  const S *__restrict__ gA = a.A;
+  const S *__restrict__ gB = a.B;
+  S *__restrict__ gC = a.Cn;
+  S *__restrict__ gD = a.Dn;
+  const uint8_t *__restrict__ gpos = a.pos;
+  const uint8_t *__restrict__ gmat = a.mat;
+
+  // One z-iteration: first level at slice k (rows 1..10), second level at slice k-1 (rows 2..9).
+  // am/ac/ap: P^n at k-1, k, k+1 (12 rows); q0/q1/q2: P^{n+1} at k-2, k-1, k (rows 1..10 at
+  // index row-1); bp/bc: packed bytes at k-1 and k (rows 1..10).
+  auto body = [&](C (&am)[NA], C (&ac)[NA], C (&ap)[NA], C (&q0)[NQ], C (&q1)[NQ], C (&q2)[NQ],
+                  uint32_t (&bp)[NB], uint32_t (&bc)[NB], int k) {
+    const bool upd1 = k >= 1 && k <= Z - 2;
+    if (k + 1 <= Z - 1) {
+      const S *An = gA + (size_t)(k + 1) * XY;
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ap[r] = Fmt<S>::val(gld(An, ro[r]));
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ap[r] = (C)0;
+    }
+    C past[NQ];
+    if (upd1) {
+      const S *Bk = gB + (size_t)k * XY;
+      const uint8_t *pk = gpos + (size_t)k * XY;
+      const uint8_t *mk = gmat + (size_t)k * XY;
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) past[j] = Fmt<S>::val(gld(Bk, ro[j + 1]));
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const uint32_t r0 = ro[2 * i + 1], r1 = ro[2 * i + 2];
+        bc[i] = (uint32_t)gld(pk, r0) | ((uint32_t)gld(mk, r0) << 8) |
+                ((uint32_t)gld(pk, r1) << 16) | ((uint32_t)gld(mk, r1) << 24);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) past[j] = (C)0;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) bc[i] = 0;
+    }
+
+    // ---- first time level: P^{n+1}[k]
+    auto level1 = [&](auto table) {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const int r = j + 1;
+        C s = ap[r] + am[r];                 // z+1, z-1
+        s = s + ac[r + 1];                   // y+1
+        s = s + ac[r - 1];                   // y-1
+        s = s + lane_next(ac[r]);            // x+1
+        s = s + lane_prev(ac[r]);            // x-1
+        const C v = node(table, pm_of(bc, j), ac[r], s, past[j]);
+        q2[j] = (upd1 && ((imask >> r) & 1u)) ? Fmt<S>::round(v) : (C)0;
+      }
+    };
+    if (any_nonair(bc))
+      level1(std::true_type());
+    else
+      level1(std::false_type());
+
+    // ---- receivers (before injection) and sources on P^{n+1}[k]
+    if (nrec != 0) {
+      const int cnt = nrec < 0 ? a.n_rec : nrec;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nrec < 0 ? t : s_rec[wave][t];
+        const int32_t *q = a.rec_xyz + 3 * id;
+        if (q[2] != k || q[0] != x) continue;
+        if (nrec < 0 && !in_box(q, xb, xb + kFO, yt, yt + kFTY, z0, z1)) continue;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          if (q[1] == yt - 1 + j)
+            a.resp[(int64_t)id * a.resp_steps + a.gather_step] = (double)q2[j] * inv_scale;
+      }
+    }
+    if (nsrc != 0 && a.inject_step >= 0) {
+      const int cnt = nsrc < 0 ? a.n_src : nsrc;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nsrc < 0 ? t : s_src[wave][t];
+        const int32_t *q = a.src_xyz + 3 * id;
+        if (q[2] != k || q[0] != x) continue;
+        C smp = (C)0;
+        if (a.inject_step < a.n_samples) {
+          const int64_t off = (int64_t)id * a.n_samples + a.inject_step;
+          if (sizeof(C) == sizeof(double))
+            smp = (C)a.sig_d[off];
+          else
+            smp = (C)a.sig_f[off];
+        }
+        smp = smp * scale;
+        const bool hard = a.src_kind[id] == 0 || a.soft_overwrite;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          if (q[1] == yt - 1 + j) q2[j] = hard ? Fmt<S>::round(smp) : Fmt<S>::round(q2[j] + smp);
+      }
+    }
+
+    // ---- store the owned part of P^{n+1}[k]
+    if (k >= z0 && k < z1) {
+      S *Ck = gC + (size_t)k * XY;
+#pragma unroll
+      for (int o = 0; o < kFTY; ++o)
+        if ((smask >> o) & 1u) gst(Ck, ro[o + 2], Fmt<S>::raw(q2[o + 1]));
+    }
+
+    // ---- second time level: P^{n+2}[k-1] on the owned rows
+    if (k - 1 >= z0) {
+      S *Dk = gD + (size_t)(k - 1) * XY;
+      auto level2 = [&](auto table) {
+#pragma unroll
+        for (int o = 0; o < kFTY; ++o) {
+          const int j = o + 1;
+          C s = q2[j] + q0[j];               // z+1, z-1
+          s = s + q1[j + 1];                 // y+1
+          s = s + q1[j - 1];                 // y-1
+          s = s + lane_next(q1[j]);          // x+1
+          s = s + lane_prev(q1[j]);          // x-1
+          C v = node(table, pm_of(bp, j), q1[j], s, am[o + 2]);
+          v = ((imask >> (o + 2)) & 1u) ? v : (C)0;
+          if ((smask >> o) & 1u) gst(Dk, ro[o + 2], Fmt<S>::raw(v));
+        }
+      };
+      if (any_nonair(bp))
+        level2(std::true_type());
+      else
+        level2(std::false_type());
+    }
+  };
+
+  C A0[NA], A1[NA], A2[NA];
+  C Q0[NQ], Q1[NQ], Q2[NQ];
+  uint32_t B0[NB], B1[NB], B2[NB];
+  const int k0 = z0 - 1;
+  const int k1 = z1;
+  {
+    const S *Ak = gA + (size_t)k0 * XY;
+#pragma unroll
+    for (int r = 0; r < NA; ++r) A1[r] = Fmt<S>::val(gld(Ak, ro[r]));
+    if (k0 >= 1) {
+      const S *Am = gA + (size_t)(k0 - 1) * XY;
+#pragma unroll
+      for (int r = 0; r < NA; ++r) A0[r] = Fmt<S>::val(gld(Am, ro[r]));
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) A0[r] = (C)0;
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      Q0[j] = (C)0;
+      Q1[j] = (C)0;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) B0[i] = 0;
+  }
+  // Unrolled by three so the three-slice queues rotate by renaming, not by copies.
+  for (int k = k0;;) {
+    body(A0, A1, A2, Q0, Q1, Q2, B0, B1, k);
+    if (++k > k1) break;
+    body(A1, A2, A0, Q1, Q2, Q0, B1, B2, k);
+    if (++k > k1) break;
+    body(A2, A0, A1, Q2, Q0, Q1, B2, B0, k);
+    if (++k > k1) break;
+  }
+}
+
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision) {
+  const int kFTY = precision == 1 ? FusedTile<double>::TY : FusedTile<float>::TY;
+  FusedGeom g;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  const int zi = Z - 2;
+  g.zc = zc < 1 ? 1 : (zc > zi ? zi : zc);
+  g.grid = dim3((X + kFO - 1) / kFO, (Y + kFTY * kFWaves - 1) / (kFTY * kFWaves),
+                (zi + g.zc - 1) / g.zc);
+  g.block = dim3(64 * kFWaves, 1, 1);
+  return g;
+}
+
+template <typename S, typename CT, typename C>
+static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a,
+                               hipStream_t stream) {
+  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C>), g.grid, g.block, 0, stream, a, g.X, g.Y,
+                     g.Z, g.zc);
+  return hipGetLastError();
+}
+
+hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
+                              hipStream_t stream) {
+  return launch_fused(g, a, stream);
+}
+hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,
+                              hipStream_t stream) {
+  return launch_fused(g, a, stream);
+}
+hipError_t launch_update2_f16(const FusedGeom &g, const Fused2Args<h16, CoefF, float> &a,
+                              hipStream_t stream) {
+  return launch_fused(g, a, stream);
+}
+hipError_t launch_update2_bf16(const FusedGeom &g, const Fused2Args<b16, CoefF, float> &a,
+                               hipStream_t stream) {
+  return launch_fused(g, a, stream);
+}
+
+}  // namespace pfdtd

@@ -73,6 +73,52 @@ hipError_t launch_update_f16(const UpdateGeom &g, const UpdateArgs<h16, CoefF, f
 hipError_t launch_update_bf16(const UpdateGeom &g, const UpdateArgs<b16, CoefF, float> &a,
                               hipStream_t stream);
 
+// Two-time-level update (temporal blocking): reads P^n (A), P^{n-1} (B), writes P^{n+1} (Cn)
+// with the sources of step n+1 injected, and P^{n+2} (Dn); gathers receivers of step n from
+// P^{n+1} before the injection. Coordinates of sources/receivers outside the grid are
+// (-2^30, -2^30, -2^30).
+template <typename S, typename CT, typename C>
+struct Fused2Args {
+  const uint8_t *pos;
+  const uint8_t *mat;
+  const S *A;
+  const S *B;
+  S *Cn;
+  S *Dn;
+  const CT *coef;
+  CT air;
+  int air_uniform;
+  C lam2;
+  const int32_t *src_xyz;
+  const int32_t *src_kind;
+  int n_src;
+  const float *sig_f;
+  const double *sig_d;
+  int n_samples;
+  int inject_step;
+  int soft_overwrite;
+  int scale_exp;
+  const int32_t *rec_xyz;
+  int n_rec;
+  double *resp;
+  int resp_steps;
+  int gather_step;
+};
+struct FusedGeom {
+  int X, Y, Z;
+  int zc;
+  dim3 grid, block;
+};
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision);
+hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
+                              hipStream_t stream);
+hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,
+                              hipStream_t stream);
+hipError_t launch_update2_f16(const FusedGeom &g, const Fused2Args<h16, CoefF, float> &a,
+                              hipStream_t stream);
+hipError_t launch_update2_bf16(const FusedGeom &g, const Fused2Args<b16, CoefF, float> &a,
+                               hipStream_t stream);
+
 // Receiver gather of time index `gather_step` (skipped when < 0) followed by injection of
 // source samples of time index `inject_step` (skipped when < 0) into the same field.
 struct InjectArgs {

@@ -8,72 +8,9 @@
 // separately rounded IEEE operation in the reference's operand order; with the per-(K, m)
 // coefficients built on the host in the same order, results are bit-identical to a CPU
 // evaluation of the reference expression.
-#include "pfdtd_internal.h"
+#include "pfdtd_device.h"
 
 namespace pfdtd {
-
-// ---------------------------------------------------------------------------------------
-// Field storage formats
-// ---------------------------------------------------------------------------------------
-template <typename S>
-struct Fmt;
-
-template <>
-struct Fmt<float> {
-  typedef float C;
-  static __device__ __forceinline__ float ld(const float *p, int64_t i) { return p[i]; }
-  static __device__ __forceinline__ void st(float *p, int64_t i, float v) { p[i] = v; }
-};
-
-template <>
-struct Fmt<double> {
-  typedef double C;
-  static __device__ __forceinline__ double ld(const double *p, int64_t i) { return p[i]; }
-  static __device__ __forceinline__ void st(double *p, int64_t i, double v) { p[i] = v; }
-};
-
-template <>
-struct Fmt<h16> {
-  typedef float C;
-  static __device__ __forceinline__ float ld(const h16 *p, int64_t i) {
-    return (float)__builtin_bit_cast(_Float16, p[i].bits);
-  }
-  static __device__ __forceinline__ void st(h16 *p, int64_t i, float v) {
-    // The value must be rounded to FP32 first and then to binary16 (round to nearest even):
-    // without the register barrier the compiler folds a producing multiply into
-    // v_fma_mixlo_f16, which rounds the exact product once, straight to binary16.
-    asm volatile("" : "+v"(v));
-    p[i].bits = __builtin_bit_cast(uint16_t, (_Float16)v);
-  }
-};
-
-template <>
-struct Fmt<b16> {
-  typedef float C;
-  static __device__ __forceinline__ float ld(const b16 *p, int64_t i) {
-    return __uint_as_float((uint32_t)p[i].bits << 16);
-  }
-  static __device__ __forceinline__ void st(b16 *p, int64_t i, float v) {
-    uint32_t x = __float_as_uint(v);
-    if ((x & 0x7FFFFFFFu) > 0x7F800000u) {
-      p[i].bits = (uint16_t)((x >> 16) | 0x40u);  // quiet NaN
-      return;
-    }
-    x += 0x7FFFu + ((x >> 16) & 1u);  // round to nearest even
-    p[i].bits = (uint16_t)(x >> 16);
-  }
-};
-
-template <typename C>
-struct CoefOf;
-template <>
-struct CoefOf<float> {
-  typedef CoefF T;
-};
-template <>
-struct CoefOf<double> {
-  typedef CoefD T;
-};
 
 // ---------------------------------------------------------------------------------------
 // SRL_FORWARD update: row-segment tiles marching short z-chunks

@@ -60,6 +60,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_get_kernel_timing": [_p, _pdbl, _pi64],
     "pfdtd_set_poll_interval": [_p, C.c_int],
     "pfdtd_set_update_tiling": [_p, C.c_int, C.c_int],
+    "pfdtd_set_time_levels": [_p, C.c_int, C.c_int],
 }
 _RESTYPE = {"pfdtd_last_error": C.c_char_p}
 

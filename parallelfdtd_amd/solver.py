@@ -123,6 +123,9 @@ class Solver:
     def set_update_tiling(self, rows: int = 1, zchunk: int = 8) -> None:
         self._c("pfdtd_set_update_tiling", rows, zchunk)
 
+    def set_time_levels(self, levels: int = 0, zchunk: int = 32) -> None:
+        self._c("pfdtd_set_time_levels", levels, zchunk)
+
     def set_poll_interval(self, steps: int) -> None:
         self._c("pfdtd_set_poll_interval", steps)
 

@@ -216,6 +216,7 @@ def test_tiling_invariance(gpu, rows, zchunk, prec):
                                     scale_exp=scale, want_fields=True)
     with Solver() as s:
         s.load_scene(sc)
+        s.set_time_levels(1)
         s.set_update_tiling(rows, zchunk)
         if prec == N.F16:
             s.prepare_fields(N.F16, scale)
