@@ -2,6 +2,8 @@
 
 * ``parallelfdtd_amd/libpfdtd.so`` -- the HIP kernels and the C ABI (include/pfdtd.h),
   compiled by hipcc for gfx950 only.
+* ``parallelfdtd_amd/libpfdtd_app.so`` -- the FDTD::App facade (plain C++ over the C ABI).
+* ``parallelfdtd_amd/libPyFDTD*.so`` -- pybind11 module with the reference's Python surface.
 * ``oracle/liboracle.so`` -- the CPU parity oracle (test infrastructure), compiled by gcc.
 
 Run ``python -m parallelfdtd_amd.build`` or call :func:`build_all`.
@@ -53,6 +55,34 @@ def build_lib(out: Path | None = None) -> Path:
     return out
 
 
+def build_app() -> Path:
+    """Compile the FDTD::App facade (HIP-free C++, plain g++) into libpfdtd_app.so."""
+    out = PKG / "libpfdtd_app.so"
+    tmp = out.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", str(ROOT / "include"),
+          str(CSRC / "app" / "fdtd_app.cpp"), "-L", str(PKG), "-lpfdtd",
+          "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
+def build_pymodule() -> Path:
+    """Compile the pybind11 module libPyFDTD (the reference's Python surface)."""
+    import sysconfig
+
+    import pybind11
+
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = PKG / f"libPyFDTD{suffix}"
+    tmp = PKG / "libPyFDTD.so.tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+          "-I", str(ROOT / "include"), "-I", str(CSRC / "app"), "-I", pybind11.get_include(),
+          "-I", sysconfig.get_paths()["include"], str(CSRC / "app" / "pyfdtd.cpp"),
+          "-L", str(PKG), "-lpfdtd_app", "-lpfdtd", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle() -> Path:
     """Compile the CPU oracle (oracle/Makefile)."""
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
@@ -61,10 +91,13 @@ def build_oracle() -> Path:
 
 def build_all() -> None:
     build_lib()
+    build_app()
+    build_pymodule()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "oracle": build_oracle}[w]()
+        {"all": build_all, "lib": build_lib, "app": build_app, "py": build_pymodule,
+         "oracle": build_oracle}[w]()

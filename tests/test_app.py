@@ -1,0 +1,168 @@
+"""The FDTD::App facade through the libPyFDTD module (the reference's Python surface,
+src/AppPy.cpp:96-134): host semantics on the CPU against the oracle, and reference-style
+runs on the GPU against the oracle."""
+import math
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from parallelfdtd_amd import rooms
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "parallelfdtd_amd"))
+pf = pytest.importorskip("libPyFDTD")
+
+LAM = math.sqrt(1.0 / 3.0)
+
+
+def test_python_surface_has_reference_methods():
+    # the 25 names bound by AppPy.cpp:107-132
+    names = ["initializeDevices", "initializeGeometryFromFile", "initializeGeometryPy",
+             "setLayerIndices", "addSource", "addSourceDataFloat", "addSourceDataDouble",
+             "addReceiver", "addSurfaceMaterials", "setSpatialFs", "setNumSteps", "setUpdateType",
+             "setUniform", "runVisualization", "runSimulation", "runCapture", "setUniformMaterial",
+             "getResponse", "getResponseDouble", "forcePartitionTo", "addSliceToCapture",
+             "setDouble", "setCapturedB", "close", "getMvox", "getNumElems"]
+    app = pf.App()
+    assert all(hasattr(app, n) for n in names)
+
+
+def test_parameter_vector_and_dx():
+    app = pf.App()
+    app.setUpdateType(0)  # lambda = sqrt(1/3), as SimulationParameters::setUpdateType sets it
+    f = np.zeros(4, np.float32)
+    oracle.lib().orc_param_vector_f(LAM, 0, f.ctypes.data)
+    assert np.array_equal(np.array(app.getParameterVector(), np.float32), f)
+    app.setSpatialFs(2000)
+    assert app.getDx() == oracle.lib().orc_dx(344.0, 2000, LAM)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("signal", [0, 1, 2, 3])
+def test_source_samples_match_oracle(kind, signal):
+    app = pf.App()
+    app.setUpdateType(0)
+    app.setSpatialFs(48000)
+    data = np.linspace(-1, 1, 37).astype(np.float32)
+    app.addSourceDataFloat(data.tolist(), len(data), 1)
+    app.addSourceDataDouble(data.astype(np.float64).tolist(), len(data), 1)
+    h = np.array([0.0, 0.0, -1 / 3, 0.0, 1 / 9, 0.0, -0.0617], np.float32)
+    app.setGridIr(h.tolist())
+    app.addSource(0.5, 0.25, 0.75, kind, signal, 0)
+    L = oracle.lib()
+    for n in range(60):
+        if kind == 2:
+            ef = L.orc_transparent_sample_f(signal, n, 48000, data.ctypes.data, len(data), h.ctypes.data, len(h))
+            d64 = data.astype(np.float64)
+            ed = L.orc_transparent_sample_d(signal, n, 48000, d64.ctypes.data, len(data), h.ctypes.data, len(h))
+        else:
+            ef = L.orc_source_sample_f(signal, n, 48000, data.ctypes.data, len(data))
+            d64 = data.astype(np.float64)
+            ed = L.orc_source_sample_d(signal, n, 48000, d64.ctypes.data, len(data))
+        assert np.float32(app.getSourceSample(0, n)) == np.float32(ef)
+        assert app.getSourceSampleDouble(0, n) == ed
+
+
+def test_element_coordinates_match_oracle():
+    app = pf.App()
+    app.setUpdateType(0)
+    app.setSpatialFs(20000)
+    pts = [(0.31, 0.47, 0.123), (1.0, 0.0, 2.5)]
+    for p in pts:
+        app.addSource(*p, 0, 0, 0)
+        app.addReceiver(*p)
+    out = np.zeros(3, np.int32)
+    for i, p in enumerate(pts):
+        oracle.lib().orc_element_idx(*p, 20000, 344.0, LAM, 1, out.ctypes.data)
+        assert app.getSourceElementCoordinates(i) == tuple(out)
+        assert app.getReceiverElementCoordinates(i) == tuple(out)
+
+
+def test_out_of_range_and_missing_domain():
+    app = pf.App()
+    with pytest.raises(IndexError):
+        app.getSourceSample(3, 0)
+    with pytest.raises(RuntimeError):
+        app.runVisualization()
+
+
+# ------------------------------------------------------------------------------------------
+# GPU: reference-style runs
+# ------------------------------------------------------------------------------------------
+def _room_app(fs, n_steps, double=False, box=(40, 32, 24)):
+    lab = rooms.box_room(*box, floor=0, ceiling=1, walls=1)
+    lab[8:14, 10:20, 12:22] = 3  # an obstacle of material 2
+    betas = [0.02, 0.3, 0.55]
+    app = pf.App()
+    app.initializeDevices()
+    app.setUpdateType(0)
+    app.setSpatialFs(fs)
+    app.setNumSteps(n_steps)
+    app.setDouble(double)
+    for b in betas:
+        app.addSurfaceMaterials([b] * 20, 1, 20)
+    app.initializeVoxelLabels(lab)
+    return app, lab, np.array(betas, np.float32)
+
+
+def _oracle_responses(app, lab, betas, n_steps, precision):
+    pos, mat, _ = oracle.classify(lab)
+    ns, nr = app.getNumSources(), app.getNumReceivers()
+    src = np.array([app.getSourceElementCoordinates(i) for i in range(ns)], np.int32)
+    rec = np.array([app.getReceiverElementCoordinates(i) for i in range(nr)], np.int32)
+    get = app.getSourceSampleDouble if precision == "f64" else app.getSourceSample
+    sig = np.array([[get(i, n) for n in range(n_steps)] for i in range(ns)])
+    table = np.repeat(betas[:, None], 20, axis=1)
+    return oracle.run(pos, mat, table, src, np.array([1] * ns, np.int32), sig, rec, n_steps,
+                      precision=precision)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("double", [False, True])
+def test_run_simulation_matches_oracle(gpu, double):
+    n_steps = 300
+    app, lab, betas = _room_app(fs=40000, n_steps=n_steps, double=double)
+    app.addSource(0.15, 0.12, 0.10, 1, 1, 0)    # soft Gaussian (input type 1)
+    app.addReceiver(0.2, 0.15, 0.12)
+    app.addReceiver(0.05, 0.05, 0.05)
+    app.runSimulation()
+    prec = "f64" if double else "f32"
+    ref = _oracle_responses(app, lab, betas, n_steps, prec)
+    got = np.array([(app.getResponseDouble if double else app.getResponse)(r) for r in range(2)])
+    assert np.array_equal(got, ref)
+    assert app.getResponseSize() == 2 * n_steps
+    assert app.getNumElems() == lab.size and app.getTimePerStep() > 0
+    app.close()
+
+
+@pytest.mark.gpu
+def test_run_capture_writes_slices_and_meshes(gpu, tmp_path):
+    n_steps = 40
+    app, lab, betas = _room_app(fs=40000, n_steps=n_steps)
+    app.addSource(0.15, 0.12, 0.10, 1, 1, 0)
+    app.addReceiver(0.2, 0.15, 0.12)
+    app.setCaptureDirectory(str(tmp_path))
+    app.addSliceToCapture(10, 20, 0)
+    app.addSliceToCapture(12, 30, 1)
+    app.addSliceToCapture(7, 30, 2)
+    app.addSliceToCapture(999, 30, 0)  # out of range: logged and skipped, as the reference does
+    app.addMeshToCapture(25)
+    app.runCapture()
+    ref = _oracle_responses(app, lab, betas, n_steps, "f32")
+    assert np.array_equal(app.getResponse(0), ref[0])
+    files = sorted(os.listdir(tmp_path))
+    assert files == ["capture_0_20_10.tga", "capture_1_30_12.tga", "capture_2_30_7.tga"]
+    X, Y, Z = lab.shape[2], lab.shape[1], lab.shape[0]
+    raw = (tmp_path / "capture_0_20_10.tga").read_bytes()
+    assert len(raw) == 18 + 4 * X * Y and raw[2] == 2 and raw[16] == 32
+    assert app.getNumberOfMeshCaptures() == 1
+    pos, mat, _ = oracle.classify(lab)
+    src = np.array([app.getSourceElementCoordinates(0)], np.int32)
+    sig = np.array([[app.getSourceSample(0, n) for n in range(25)]])
+    _, P, _ = oracle.run(pos, mat, np.repeat(betas[:, None], 20, axis=1), src, np.array([1]), sig,
+                         src[:0], 25, want_fields=True)
+    assert np.array_equal(app.getMeshCaptureAt(0), P)
+    app.close()
