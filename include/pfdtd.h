@@ -117,6 +117,18 @@ int pfdtd_set_sources(pfdtd_ctx *ctx, int n_src, const int32_t *xyz, const int32
  * SimulationParameters.cpp:213-224). Receivers outside the grid record 0 (kernels3d.cu:168). */
 int pfdtd_set_receivers(pfdtd_ctx *ctx, int n_rec, const int32_t *xyz);
 
+/* ---- source batching ---------------------------------------------------------------- */
+/* Advance `slots` independent source configurations on the same domain in one launch per
+ * step (power of two, 1..256; 1 = unbatched, the default). Fields then hold `slots` values per
+ * voxel, slot innermost, and every source and receiver belongs to one slot (default 0; set
+ * them after pfdtd_set_sources / pfdtd_set_receivers). Each slot evolves exactly as an
+ * unbatched run with only its own sources would. No reference counterpart: the reference runs
+ * one source configuration per simulation (App::runSimulation, src/App.cpp:308-348). Batched
+ * runs use the single-level update; slice/field capture is unavailable (raw fields are). */
+int pfdtd_set_batch(pfdtd_ctx *ctx, int slots);
+int pfdtd_set_source_slots(pfdtd_ctx *ctx, int n_src, const int32_t *slot);
+int pfdtd_set_receiver_slots(pfdtd_ctx *ctx, int n_rec, const int32_t *slot);
+
 /* ---- running ------------------------------------------------------------------------ */
 /* Allocate (if needed) and zero the two pressure fields in the given precision.
  * scale_exp: fields are stored multiplied by 2^scale_exp (F16/BF16 only; 0 otherwise).
@@ -176,7 +188,7 @@ int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
  * Results are bit-identical to single steps. No reference counterpart: the reference's loop
  * (kernels3d.cu:83-191) launches one update per step. */
 int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
-/* Update-kernel tiling: y rows per thread (1, 2 or 4; default 2) and slices per z-chunk
+/* Update-kernel tiling: y rows per thread (1, 2, 4 or 8; default 2) and slices per z-chunk
  * (default 8). Results do not depend on it (every voxel's arithmetic is the same); it only
  * changes the streaming pattern. Replaces the fixed 32x4 block of App::initializeMesh
  * (src/App.cpp:193) and the one-block-per-slice grid of launchFDTD3d (kernels3d.cu:42-45). */

@@ -113,9 +113,10 @@ def run(pos, mat, table, src_xyz, src_kind, signals, rec_xyz, n_steps, *, precis
     rec_xyz = np.ascontiguousarray(rec_xyz, np.int32).reshape(-1, 3)
     n_src, n_rec = src_xyz.shape[0], rec_xyz.shape[0]
     sig = np.zeros((n_src, n_steps), dt)
-    s = np.asarray(signals).reshape(n_src, -1)
-    k = min(n_steps, s.shape[1])
-    sig[:, :k] = s[:, :k].astype(dt)
+    if n_src:
+        s = np.asarray(signals).reshape(n_src, -1)
+        k = min(n_steps, s.shape[1])
+        sig[:, :k] = s[:, :k].astype(dt)
     sig = np.ascontiguousarray(sig)
     resp = np.zeros((n_rec, n_steps), dt)
     P = np.zeros((Z, Y, X), dt) if want_fields else None

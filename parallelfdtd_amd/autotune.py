@@ -1,0 +1,109 @@
+"""Autotuner for batched runs: sweeps the batch width B (sources advanced per launch, one slot
+each, fields interleaved source-innermost), the update tile (y rows per thread; a 256-thread
+block covers 256/B x-columns of one row band) and the z-chunk length, timing a short run of
+each candidate on the real domain.
+
+Throughput is per source: voxel-updates/s summed over the B slots,
+  per_source_gvox = N_nodes * B * steps / seconds / 1e9,
+with the step loop's own HIP-event time (injection and gather included). B = 1 runs the
+unbatched single-level update on the same domain as the baseline of the comparison.
+
+Run:  python -m parallelfdtd_amd.autotune --batches 1,8,16,32,64 --out tune_c4.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+from . import native as N
+from . import rooms
+from .solver import Solver
+
+BYTES_F32 = {"per_voxel": 2, "per_slot": 12}  # 2 + 12*B bytes per voxel-step, FP32 fields
+
+
+def load_batch(s: Solver, scene, B: int, rec_per_src: int) -> None:
+    """Domain of `scene` with its first B sources, one per slot, and their receivers."""
+    s.set_sources(scene.src_xyz[:B], scene.src_kind[:B], signal_f64=scene.signals[:B],
+                  signal_f32=scene.signals[:B].astype(np.float32))
+    s.set_receivers(scene.rec_xyz[: B * rec_per_src])
+    s.set_batch(B)
+    s.set_source_slots(np.arange(B, dtype=np.int32))
+    s.set_receiver_slots(np.repeat(np.arange(B, dtype=np.int32), rec_per_src))
+
+
+def time_config(s: Solver, steps: int, warmup: int, precision: int) -> float:
+    """Milliseconds per step of the step loop (after a warm-up run)."""
+    if warmup:
+        s.run(warmup, precision)
+    done, ms = s.run(steps, precision)
+    if done != steps:
+        raise RuntimeError(f"run stopped after {done} of {steps} steps")
+    return ms / steps
+
+
+def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 16),
+          steps: int = 20, warmup: int = 3, rec_per_src: int = 8, device: int = 0,
+          log=print) -> dict:
+    nodes = scene.n_nodes
+    if max(batches) > len(scene.src_xyz):
+        raise ValueError(f"scene has {len(scene.src_xyz)} sources, batch {max(batches)} asked")
+    out = {"scene": scene.name, "nodes": nodes, "steps": steps, "candidates": [], "best": {}}
+    with Solver(device) as s:
+        s.set_domain_labels(scene.labels)
+        s.set_materials(scene.material_table())
+        s.set_time_levels(1)
+        for B in batches:
+            load_batch(s, scene, B, rec_per_src)
+            best = None
+            for ry in rows:
+                for zc in zchunks:
+                    s.set_update_tiling(ry, zc)
+                    ms = time_config(s, steps, warmup, N.F32)
+                    per_src = nodes * B / (ms * 1e-3) / 1e9
+                    gbs = nodes * (BYTES_F32["per_voxel"] + BYTES_F32["per_slot"] * B) / (ms * 1e-3) / 1e9
+                    cand = {"B": B, "rows": ry, "zchunk": zc, "ms_per_step": round(ms, 4),
+                            "per_source_gvox": round(per_src, 2), "alg_GBps": round(gbs, 1)}
+                    out["candidates"].append(cand)
+                    log(json.dumps(cand))
+                    if best is None or per_src > best["per_source_gvox"]:
+                        best = cand
+            out["best"][str(B)] = best
+    base = out["best"].get("1")
+    if base:
+        for k, v in out["best"].items():
+            v["per_source_speedup_vs_B1"] = round(v["per_source_gvox"] / base["per_source_gvox"], 3)
+    top = max(out["best"].values(), key=lambda c: c["per_source_gvox"])
+    out["choice"] = {k: top[k] for k in ("B", "rows", "zchunk")}
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--batches", default="1,8,16,32,64")
+    ap.add_argument("--rows", default="1,2,4,8")
+    ap.add_argument("--zchunks", default="4,8,16")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=1, help="divide every room dimension")
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    batches = [int(v) for v in args.batches.split(",")]
+    scene = rooms.scene_c4(n_sources=max(batches), n_steps=args.steps, scale=args.scale)
+    rep = sweep(scene, batches, [int(v) for v in args.rows.split(",")],
+                [int(v) for v in args.zchunks.split(",")], args.steps, args.warmup,
+                log=lambda m: print(m, file=sys.stderr, flush=True))
+    text = json.dumps(rep)
+    if args.out:
+        Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+        Path(args.out).write_text(text)
+    print(json.dumps({"scene": rep["scene"], "best": rep["best"], "choice": rep["choice"]}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -102,6 +102,9 @@ struct pfdtd_ctx {
   double *d_sig_d = nullptr;
   int n_rec = 0;
   std::vector<int32_t> rec_xyz;
+  // batch: fields hold 2^lgB values per voxel (slot innermost); per-source/receiver slot
+  int lgB = 0;
+  std::vector<int32_t> src_slot, rec_slot;
   int64_t *d_rec_idx = nullptr;
   int32_t *d_rec_xyz = nullptr;
 
@@ -144,6 +147,12 @@ int64_t lin_idx(const pfdtd_ctx *c, const int32_t *q) {
   if (q[0] < 0 || q[1] < 0 || q[2] < 0 || q[0] >= c->X || q[1] >= c->Y || q[2] >= c->Z)
     return -1;
   return ((int64_t)q[2] * c->Y + q[1]) * c->X + q[0];
+}
+
+// Element index of a source/receiver in the (possibly batched) field, -1 outside the grid.
+int64_t elem_idx(const pfdtd_ctx *c, const int32_t *q, int slot) {
+  const int64_t v = lin_idx(c, q);
+  return v < 0 ? -1 : (v << c->lgB) + slot;
 }
 
 constexpr int32_t kOutside = -(1 << 30);
@@ -269,7 +278,7 @@ int finish_domain(pfdtd_ctx *c) {
   // source/receiver linear indices depend on the grid
   if (c->n_src) {
     std::vector<int64_t> idx(c->n_src);
-    for (int s = 0; s < c->n_src; ++s) idx[s] = lin_idx(c, &c->src_xyz[3 * s]);
+    for (int s = 0; s < c->n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], c->src_slot[s]);
     HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src,
                       hipMemcpyHostToDevice));
     rc = upload_xyz(c, c->src_xyz, c->d_src_xyz);
@@ -277,7 +286,7 @@ int finish_domain(pfdtd_ctx *c) {
   }
   if (c->n_rec) {
     std::vector<int64_t> idx(c->n_rec);
-    for (int r = 0; r < c->n_rec; ++r) idx[r] = lin_idx(c, &c->rec_xyz[3 * r]);
+    for (int r = 0; r < c->n_rec; ++r) idx[r] = elem_idx(c, &c->rec_xyz[3 * r], c->rec_slot[r]);
     HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * c->n_rec,
                       hipMemcpyHostToDevice));
     rc = upload_xyz(c, c->rec_xyz, c->d_rec_xyz);
@@ -315,8 +324,8 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
   if ((precision == PFDTD_F32 || precision == PFDTD_F64) && scale_exp != 0)
     return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
   if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
-  const size_t bytes = es * (size_t)c->N;
-  const bool want4 = levels_for(c, precision) == 2;
+  const size_t bytes = es * ((size_t)c->N << c->lgB);
+  const bool want4 = c->lgB == 0 && levels_for(c, precision) == 2;
   if (c->d_P && c->field_bytes == bytes && (!want4 || c->d_P2)) {
     c->precision = precision;
     c->scale_exp = scale_exp;
@@ -409,7 +418,7 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
 // The two-level update needs the spare pair and no source on the never-updated slices
 // z = 0 and z = Z-1 (their values would have to be carried between the four buffers).
 bool fused_eligible(const pfdtd_ctx *c) {
-  if (levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2) return false;
+  if (c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2) return false;
   for (int s = 0; s < c->n_src; ++s) {
     const int32_t *q = &c->src_xyz[3 * s];
     if (lin_idx(c, q) >= 0 && (q[2] == 0 || q[2] == c->Z - 1)) return false;
@@ -682,6 +691,7 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
   c->n_samples = 0;
   c->src_xyz.assign(xyz, xyz + 3 * (size_t)n_src);
   c->src_kind.assign(kind, kind + n_src);
+  c->src_slot.assign((size_t)n_src, 0);
   if (n_src == 0) return PFDTD_OK;
   const size_t ns = (size_t)n_src * (size_t)n_samples;
   std::vector<float> sf(ns);
@@ -692,7 +702,7 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
   }
   std::vector<int64_t> idx(n_src, -1);
   if (c->has_domain)
-    for (int s = 0; s < n_src; ++s) idx[s] = lin_idx(c, &c->src_xyz[3 * s]);
+    for (int s = 0; s < n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], 0);
   HIP_TRY(hipMalloc((void **)&c->d_src_idx, sizeof(int64_t) * n_src));
   HIP_TRY(hipMalloc((void **)&c->d_src_kind, sizeof(int32_t) * n_src));
   HIP_TRY(hipMalloc((void **)&c->d_src_xyz, sizeof(int32_t) * 3 * n_src));
@@ -723,11 +733,12 @@ int pfdtd_set_receivers(pfdtd_ctx *c, int n_rec, const int32_t *xyz) {
   dfree(c->d_rec_xyz);
   c->n_rec = 0;
   c->rec_xyz.assign(xyz, xyz + 3 * (size_t)n_rec);
+  c->rec_slot.assign((size_t)n_rec, 0);
   c->resp_steps = 0;
   if (n_rec == 0) return PFDTD_OK;
   std::vector<int64_t> idx(n_rec, -1);
   if (c->has_domain)
-    for (int r = 0; r < n_rec; ++r) idx[r] = lin_idx(c, &c->rec_xyz[3 * r]);
+    for (int r = 0; r < n_rec; ++r) idx[r] = elem_idx(c, &c->rec_xyz[3 * r], 0);
   HIP_TRY(hipMalloc((void **)&c->d_rec_idx, sizeof(int64_t) * n_rec));
   HIP_TRY(hipMalloc((void **)&c->d_rec_xyz, sizeof(int32_t) * 3 * n_rec));
   {
@@ -773,7 +784,7 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   if (steps_done) *steps_done = 0;
   c->upd_ms = 0.0;
   c->upd_launches = 0;
-  const UpdateGeom g = plan_update((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc);
+  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc, c->lgB);
   const int bundle = c->poll_interval > 0 ? c->poll_interval : kProgressMod;
   if (c->kernel_timing) {
     rc = ensure_events(c, 2 * (size_t)bundle);
@@ -873,7 +884,7 @@ int pfdtd_step(pfdtd_ctx *c, int step, int direction) {
   // launchFDTD3dStep: inject this step's samples, update, flip unless the direction changed,
   // then read receivers into response[step] (kernels3d.cu:390-475).
   if (step >= 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, step), c->stream));
-  const UpdateGeom g = plan_update((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc);
+  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc, c->lgB);
   HIP_TRY(launch_update_any(c, g));
   if (c->last_direction == direction) std::swap(c->d_P, c->d_Pp);
   c->last_direction = direction;
@@ -916,6 +927,7 @@ int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_o
                         uint8_t *pos_out) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if (c->lgB) return fail(PFDTD_E_STATE, "capture of batched fields is not supported (use pfdtd_get_fields_raw)");
   if (!c->d_P || c->precision < 0) return fail(PFDTD_E_STATE, "fields not prepared");
   if (!p_out || !pos_out) return fail(PFDTD_E_ARG, "null output");
   int64_t lim, U, V;
@@ -947,6 +959,7 @@ int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_o
 int pfdtd_capture_field(pfdtd_ctx *c, float *out, int64_t capacity) {
   int rc = check_ctx(c);
   if (rc) return rc;
+  if (c->lgB) return fail(PFDTD_E_STATE, "capture of batched fields is not supported (use pfdtd_get_fields_raw)");
   if (!c->d_P || c->precision < 0) return fail(PFDTD_E_STATE, "fields not prepared");
   if (!out || capacity < c->N) return fail(PFDTD_E_ARG, "output buffer too small");
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -990,8 +1003,8 @@ int pfdtd_get_kernel_timing(pfdtd_ctx *c, double *update_ms_total, int64_t *upda
 
 int pfdtd_set_update_tiling(pfdtd_ctx *c, int rows, int zchunk) {
   if (!c) return fail(PFDTD_E_ARG, "null context");
-  if (rows != 1 && rows != 2 && rows != 4)
-    return fail(PFDTD_E_ARG, "rows per thread %d not in {1,2,4}", rows);
+  if (rows != 1 && rows != 2 && rows != 4 && rows != 8)
+    return fail(PFDTD_E_ARG, "rows per thread %d not in {1,2,4,8}", rows);
   if (zchunk < 1 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 1..65535", zchunk);
   c->ry = rows;
   c->zc = zchunk;
@@ -1010,6 +1023,58 @@ int pfdtd_set_time_levels(pfdtd_ctx *c, int levels, int zchunk) {
   c->levels = levels;
   c->fzc = zchunk;
   return PFDTD_OK;
+}
+
+// Re-derives the device element indices of sources and receivers (grid, batch or slots changed).
+static int refresh_indices(pfdtd_ctx *c) {
+  if (c->n_src) {
+    std::vector<int64_t> idx(c->n_src);
+    for (int s = 0; s < c->n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], c->src_slot[s]);
+    HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src, hipMemcpyHostToDevice));
+  }
+  if (c->n_rec) {
+    std::vector<int64_t> idx(c->n_rec);
+    for (int r = 0; r < c->n_rec; ++r) idx[r] = elem_idx(c, &c->rec_xyz[3 * r], c->rec_slot[r]);
+    HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * c->n_rec, hipMemcpyHostToDevice));
+  }
+  return PFDTD_OK;
+}
+
+int pfdtd_set_batch(pfdtd_ctx *c, int slots) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  int lg = 0;
+  while ((1 << lg) < slots && lg < 9) ++lg;
+  if (slots < 1 || slots > 256 || (1 << lg) != slots)
+    return fail(PFDTD_E_ARG, "batch width %d is not a power of two in 1..256", slots);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (lg != c->lgB) free_fields(c);
+  c->lgB = lg;
+  for (int32_t &s : c->src_slot) s = 0;
+  for (int32_t &r : c->rec_slot) r = 0;
+  return refresh_indices(c);
+}
+
+int pfdtd_set_source_slots(pfdtd_ctx *c, int n_src, const int32_t *slot) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_src != c->n_src || (n_src && !slot)) return fail(PFDTD_E_ARG, "expected %d source slots", c->n_src);
+  for (int s = 0; s < n_src; ++s)
+    if (slot[s] < 0 || slot[s] >= (1 << c->lgB)) return fail(PFDTD_E_ARG, "source %d: slot %d outside the batch", s, slot[s]);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->src_slot.assign(slot, slot + n_src);
+  return refresh_indices(c);
+}
+
+int pfdtd_set_receiver_slots(pfdtd_ctx *c, int n_rec, const int32_t *slot) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_rec != c->n_rec || (n_rec && !slot)) return fail(PFDTD_E_ARG, "expected %d receiver slots", c->n_rec);
+  for (int r = 0; r < n_rec; ++r)
+    if (slot[r] < 0 || slot[r] >= (1 << c->lgB)) return fail(PFDTD_E_ARG, "receiver %d: slot %d outside the batch", r, slot[r]);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->rec_slot.assign(slot, slot + n_rec);
+  return refresh_indices(c);
 }
 
 int pfdtd_set_poll_interval(pfdtd_ctx *c, int steps) {

@@ -43,9 +43,12 @@ struct UpdateGeom {
   int X, Y, Z;
   int ry;       // y rows per thread (1, 2 or 4)
   int zc;       // interior slices per block
+  int lgB;      // log2 of the batch width (0: unbatched)
   dim3 grid, block;
 };
 UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc);
+// Batched update: fields hold 2^lgB values per voxel, slot innermost.
+UpdateGeom plan_update_batch(int X, int Y, int Z, int ry, int zc, int lgB);
 
 // Arguments of one update launch: fields in storage type S, coefficients of compute type C.
 // `air` holds the coefficients shared by every air voxel (position byte 0x86); air_uniform

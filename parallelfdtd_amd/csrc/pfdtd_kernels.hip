@@ -104,6 +104,101 @@ __global__ __launch_bounds__(kTX) void srl_update_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// SRL_FORWARD update of a batch of B source configurations on one domain
+// ---------------------------------------------------------------------------------------
+// Fields hold B values per voxel, slot innermost ([voxel][slot]): a block of 256 threads
+// covers 256/B consecutive voxels of one row, lane b of each B-lane group owns slot b. All
+// B slots of a voxel share its position/material bytes and coefficients, so those bytes are
+// read once per voxel (2 + 12*B bytes per voxel-step in FP32). Each thread marches RY rows of
+// its (x, slot) column through `zc` slices with the same z-neighbour queue as the unbatched
+// kernel. lgB = log2(B), B in 1..256.
+template <typename S, int RY>
+__global__ __launch_bounds__(kTX) void srl_update_batch_kernel(
+    const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
+    S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
+    typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
+    int X, int Y, int Z, int zc, int lgB) {
+#pragma clang fp contract(off)
+  typedef typename Fmt<S>::C C;
+  const int B = 1 << lgB;
+  const int b = threadIdx.x & (B - 1);
+  const int x = blockIdx.x * (kTX >> lgB) + (int)(threadIdx.x >> lgB);
+  const int ybase = blockIdx.y * RY;
+  if (x >= X) return;
+  const int z0 = 1 + (int)blockIdx.z * zc;
+  const int z1 = min(z0 + zc, Z - 1);
+  const int64_t sX = X;
+  const int64_t XY = (int64_t)X * (int64_t)Y;
+  const int64_t eX = (int64_t)B;           // element stride of x
+  const int64_t eY = sX * B;               // of y
+  const int64_t eZ = XY * B;               // of z
+  const bool xin = (x > 0) && (x < X - 1);
+
+  bool live[RY], upd[RY];
+  int64_t row[RY];  // voxel index of (x, y_r) within a slice
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = ybase + r;
+    live[r] = y < Y;
+    upd[r] = live[r] && xin && y > 0 && y < Y - 1;
+    row[r] = (int64_t)min(y, Y - 1) * sX + x;
+  }
+  const int64_t halo_lo = (int64_t)max(ybase - 1, 0) * sX + x;
+  const int64_t halo_hi = (int64_t)min(ybase + RY, Y - 1) * sX + x;
+
+  C pm[RY], pc[RY];
+  int64_t base = (int64_t)z0 * XY;  // voxel index of the slice
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    pm[r] = Fmt<S>::ld(P, (base - XY + row[r]) * B + b);
+    pc[r] = Fmt<S>::ld(P, (base + row[r]) * B + b);
+  }
+  for (int z = z0; z < z1; ++z, base += XY) {
+    C pn[RY], past[RY], xp[RY], xm[RY];
+    unsigned pb[RY], mb[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int64_t v = base + row[r];
+      const int64_t e = v * B + b;
+      pn[r] = Fmt<S>::ld(P, e + eZ);
+      past[r] = Fmt<S>::ld(Pp, e);
+      pb[r] = pos[v];
+      mb[r] = mat[v];
+      xp[r] = Fmt<S>::ld(P, e + eX);
+      xm[r] = Fmt<S>::ld(P, e - eX);
+    }
+    const C ylo = Fmt<S>::ld(P, (base + halo_lo) * B + b);
+    const C yhi = Fmt<S>::ld(P, (base + halo_hi) * B + b);
+    (void)eY;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const C yp = (r + 1 < RY) ? pc[r + 1] : yhi;
+      const C ym = (r > 0) ? pc[r - 1] : ylo;
+      C s = pn[r] + pm[r];
+      s = s + yp;
+      s = s + ym;
+      s = s + xp[r];
+      s = s + xm[r];
+      const C ret = srl_node<C>(pb[r], mb[r], pc[r], s, past[r], coef, air, air_uniform, lam2);
+      if (live[r]) Fmt<S>::st(Pp, (base + row[r]) * B + b, upd[r] ? ret : (C)0);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      pm[r] = pc[r];
+      pc[r] = pn[r];
+    }
+  }
+}
+
+UpdateGeom plan_update_batch(int X, int Y, int Z, int ry, int zc, int lgB) {
+  UpdateGeom g = plan_update(X, Y, Z, ry, zc);
+  const int nx = kTX >> lgB;
+  g.grid.x = (X + nx - 1) / nx;
+  g.lgB = lgB;
+  return g;
+}
+
 UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc) {
   UpdateGeom g;
   g.X = X;
@@ -117,12 +212,31 @@ UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc) {
   const int nz = (zi + g.zc - 1) / g.zc;
   g.grid = dim3(bx, by, nz);
   g.block = dim3(kTX, 1, 1);
+  g.lgB = 0;
   return g;
 }
 
 template <typename S, typename CT, typename C>
 static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> &a,
                                 hipStream_t stream) {
+  if (g.lgB > 0) {
+    switch (g.ry) {
+#define PFDTD_RYB_CASE(R)                                                                    \
+  case R:                                                                                    \
+    hipLaunchKernelGGL((srl_update_batch_kernel<S, R>), g.grid, g.block, 0, stream, a.pos,   \
+                       a.mat, a.P, a.Pp, a.coef, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z, \
+                       g.zc, g.lgB);                                                         \
+    break;
+      PFDTD_RYB_CASE(1)
+      PFDTD_RYB_CASE(2)
+      PFDTD_RYB_CASE(4)
+      PFDTD_RYB_CASE(8)
+#undef PFDTD_RYB_CASE
+    default:
+      return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (g.ry) {
 #define PFDTD_RY_CASE(R)                                                                     \
   case R:                                                                                    \
@@ -132,6 +246,7 @@ static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> 
     PFDTD_RY_CASE(1)
     PFDTD_RY_CASE(2)
     PFDTD_RY_CASE(4)
+    PFDTD_RY_CASE(8)
 #undef PFDTD_RY_CASE
   default:
     return hipErrorInvalidValue;

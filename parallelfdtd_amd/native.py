@@ -59,6 +59,9 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_kernel_timing": [_p, C.c_int],
     "pfdtd_get_kernel_timing": [_p, _pdbl, _pi64],
     "pfdtd_set_poll_interval": [_p, C.c_int],
+    "pfdtd_set_batch": [_p, C.c_int],
+    "pfdtd_set_source_slots": [_p, C.c_int, _p],
+    "pfdtd_set_receiver_slots": [_p, C.c_int, _p],
     "pfdtd_set_update_tiling": [_p, C.c_int, C.c_int],
     "pfdtd_set_time_levels": [_p, C.c_int, C.c_int],
 }

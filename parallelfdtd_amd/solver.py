@@ -123,6 +123,18 @@ class Solver:
     def set_update_tiling(self, rows: int = 1, zchunk: int = 8) -> None:
         self._c("pfdtd_set_update_tiling", rows, zchunk)
 
+    def set_batch(self, slots: int) -> None:
+        self._c("pfdtd_set_batch", slots)
+        self.slots = slots
+
+    def set_source_slots(self, slots) -> None:
+        sl = np.ascontiguousarray(slots, np.int32)
+        self._c("pfdtd_set_source_slots", sl.size, N.ptr(sl))
+
+    def set_receiver_slots(self, slots) -> None:
+        sl = np.ascontiguousarray(slots, np.int32)
+        self._c("pfdtd_set_receiver_slots", sl.size, N.ptr(sl))
+
     def set_time_levels(self, levels: int = 0, zchunk: int = 32) -> None:
         self._c("pfdtd_set_time_levels", levels, zchunk)
 
@@ -174,8 +186,10 @@ class Solver:
 
     def fields_raw(self, dtype) -> tuple[np.ndarray, np.ndarray]:
         X, Y, Z = self.dims
-        p = np.zeros((Z, Y, X), dtype)
-        q = np.zeros((Z, Y, X), dtype)
+        B = getattr(self, "slots", 1)
+        shape = (Z, Y, X) if B == 1 else (Z, Y, X, B)
+        p = np.zeros(shape, dtype)
+        q = np.zeros(shape, dtype)
         self._c("pfdtd_get_fields_raw", N.ptr(p), N.ptr(q), p.nbytes)
         return p, q
 
