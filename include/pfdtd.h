@@ -195,6 +195,75 @@ int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
 int pfdtd_set_update_tiling(pfdtd_ctx *ctx, int rows, int zchunk);
 int pfdtd_get_kernel_timing(pfdtd_ctx *ctx, double *update_ms_total, int64_t *update_launches);
 
+
+/* ---- serving pool ------------------------------------------------------------------- */
+/* A pool holds many independent impulse-response jobs (rooms of different sizes, each with
+ * its own materials, sources, receivers, step count and FP32 / FP16 / BF16 storage) resident in
+ * one HBM arena, and advances all of them together: one step of every resident job of one
+ * storage type is one launch of a multi-job update kernel plus one launch of a multi-job
+ * gather/inject kernel; `bundle_steps` steps are captured into a hipGraph that is replayed
+ * until a job joins or leaves. Jobs are admitted while their buffers fit in the arena's free
+ * space and a slot is free (at most 64 resident), so a job submitted while others run starts
+ * at the next bundle and overlaps them on the device. Every job's arithmetic is that of a
+ * single-context run of the same inputs (bit-identical responses).
+ * No reference counterpart: the reference runs one simulation per App (App::runSimulation,
+ * src/App.cpp:308-348); a job's inputs and semantics are those of that call with the voxel
+ * entry point, sources as App::addSource (src/App.h) with DATA signals, and receivers as
+ * App::addReceiver. */
+typedef struct pfdtd_pool pfdtd_pool;
+
+typedef struct pfdtd_job {
+  const uint8_t *labels;   /* [Z][Y][X]: 0 inside, m+1 solid of material m; outer layer solid */
+  int32_t X, Y, Z;
+  const float *materials;  /* [n_mat][n_coef] admittances */
+  int32_t n_mat, n_coef;
+  uint32_t octave;
+  double lambda;
+  int32_t flags;           /* PFDTD_MATSLOT_LITERAL | PFDTD_SOFT_OVERWRITE */
+  int32_t n_src;
+  const int32_t *src_xyz;  /* [n_src][3] grid indices */
+  const int32_t *src_kind; /* PFDTD_SRC_* */
+  int32_t n_samples;
+  const float *signals;    /* [n_src][n_samples]; samples past the end are 0 */
+  int32_t n_rec;
+  const int32_t *rec_xyz;  /* [n_rec][3] grid indices */
+  int32_t n_steps;
+  int32_t precision;       /* PFDTD_F32, PFDTD_F16 or PFDTD_BF16 */
+  int32_t scale_exp;       /* F16/BF16 storage scale 2^scale_exp (0 for F32) */
+} pfdtd_job;
+
+/* arena_bytes <= 0: 7/8 of the device's free memory. */
+int pfdtd_pool_create(int device, int64_t arena_bytes, pfdtd_pool **out);
+int pfdtd_pool_destroy(pfdtd_pool *pool);
+/* Steps per bundle (default 32), hipGraph replay (1, default) or direct launches (0), update
+ * tiling (rows 1/2/4, z-chunk; only while empty) and the resident-job limit (1..64). */
+int pfdtd_pool_set_options(pfdtd_pool *pool, int bundle_steps, int use_graph, int rows,
+                           int zchunk, int max_jobs);
+/* Arena bytes a job occupies while resident. */
+int pfdtd_pool_job_bytes(const pfdtd_job *job, int64_t *bytes);
+int pfdtd_pool_mem(pfdtd_pool *pool, int64_t *arena_bytes, int64_t *free_bytes,
+                   int64_t *largest_free);
+/* Admits a job (its inputs are copied before the call returns). PFDTD_E_NOMEM: it does not
+ * fit now (arena space or slots); submit again after jobs are released. PFDTD_E_DOMAIN: the
+ * outer voxel layer is not solid or a label names a material outside the table. */
+int pfdtd_pool_submit(pfdtd_pool *pool, const pfdtd_job *job, int64_t *job_id);
+/* Enqueues one bundle for every resident unfinished job (returns without waiting, unless two
+ * bundles are already in flight). *n_active = jobs in the bundle (0: nothing enqueued). */
+int pfdtd_pool_advance(pfdtd_pool *pool, int *n_active);
+/* Non-blocking: ids of jobs whose last step has completed since the previous poll. */
+int pfdtd_pool_poll(pfdtd_pool *pool, int64_t *ids, int capacity, int *n_done);
+/* Waits for every enqueued bundle. */
+int pfdtd_pool_drain(pfdtd_pool *pool);
+/* phase: 0 running, 1 last bundle enqueued, 2 finished. */
+int pfdtd_pool_job_info(pfdtd_pool *pool, int64_t job_id, int *phase, int *n_rec, int *n_steps);
+/* Responses of a finished job, [n_rec][n_steps] (layout of pfdtd_get_responses_f32). */
+int pfdtd_pool_get_responses(pfdtd_pool *pool, int64_t job_id, float *out_f32, double *out_f64,
+                             int64_t capacity);
+/* Frees a finished job's arena space and slot. */
+int pfdtd_pool_release(pfdtd_pool *pool, int64_t job_id);
+int pfdtd_pool_stats(pfdtd_pool *pool, int64_t *bundles, int64_t *graph_builds,
+                     double *voxel_steps, int *resident);
+
 #ifdef __cplusplus
 }
 #endif

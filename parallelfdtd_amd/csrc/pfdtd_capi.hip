@@ -133,6 +133,55 @@ struct pfdtd_ctx {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
 };
 
+namespace pfdtd {
+
+void record_error(const char *msg) { g_last_error = msg; }
+
+// kernels3d.cu:513 (literal `mat*20*+params[3]`) vs kernels3d.cu:639 (intended)
+int64_t material_slot(unsigned m, unsigned octave, bool literal) {
+  if (literal) {
+    const unsigned base = m * 20u;
+    return (int64_t)(unsigned)((float)base * (float)octave);
+  }
+  return (int64_t)m * kCoefNum + (int64_t)octave;
+}
+
+// The per-(K, m) coefficients, evaluated in the operation order of the reference update body
+// (kernels3d.cu:513-526) so that a[K,m] multiplies exactly like the per-node expression.
+void build_coef_tables(const float *table20, int n_mat, unsigned octave, double lambda,
+                       bool literal, CoefF *cf, CoefD *cd) {
+#pragma clang fp contract(off)
+  const float lam_f = (float)lambda;                 // params[0]
+  const float lam2_f = (float)(lambda * lambda);     // params[1]
+  const double lam_d = lambda;
+  const double lam2_d = lambda * lambda;
+  for (int i = 0; i < kCoefEntries; ++i) {
+    cf[i] = CoefF{0.f, 0.f, 0.f, 0.f};
+    cd[i] = CoefD{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int K = 0; K <= 6; ++K) {
+    for (int m = 0; m < 256; ++m) {
+      const int64_t slot = material_slot((unsigned)m, octave, literal);
+      const float bm = slot < (int64_t)n_mat * kCoefNum ? table20[(size_t)slot] : 0.f;
+      const float position = (float)K;
+      const float beta = (float)(0.5 * (double)(bm * (6.f - position) * lam_f));
+      CoefF &f = cf[(size_t)K * 256 + m];
+      f.inv = 1.f / (1.f + beta);
+      f.omb = 1.f - beta;
+      f.tk = 2.f - position * lam2_f;
+      const double bmd = (double)bm;
+      const double positiond = (double)K;
+      const double betad = 0.5 * (bmd * (6.f - positiond) * lam_d);
+      CoefD &d = cd[(size_t)K * 256 + m];
+      d.inv = 1.f / (1.f + betad);
+      d.omb = 1.f - betad;
+      d.tk = 2.f - positiond * lam2_d;
+    }
+  }
+}
+
+}  // namespace pfdtd
+
 namespace {
 
 int check_ctx(pfdtd_ctx *ctx) {
@@ -169,45 +218,11 @@ int upload_xyz(const pfdtd_ctx *c, const std::vector<int32_t> &xyz, int32_t *dst
   return PFDTD_OK;
 }
 
-// kernels3d.cu:513 (literal `mat*20*+params[3]`) vs kernels3d.cu:639 (intended)
-int64_t material_slot(unsigned m, unsigned octave, bool literal) {
-  if (literal) {
-    const unsigned base = m * 20u;
-    return (int64_t)(unsigned)((float)base * (float)octave);
-  }
-  return (int64_t)m * kCoefNum + (int64_t)octave;
-}
-
-// The per-(K, m) coefficients, evaluated in the operation order of the reference update body
-// (kernels3d.cu:513-526) so that a[K,m] multiplies exactly like the per-node expression.
 void build_coefficients(const pfdtd_ctx *c, std::vector<CoefF> &cf, std::vector<CoefD> &cd) {
-#pragma clang fp contract(off)
   cf.assign(kCoefEntries, CoefF{0.f, 0.f, 0.f, 0.f});
   cd.assign(kCoefEntries, CoefD{0.0, 0.0, 0.0, 0.0});
-  const float lam_f = (float)c->lambda;                       // params[0]
-  const float lam2_f = (float)(c->lambda * c->lambda);        // params[1]
-  const double lam_d = c->lambda;
-  const double lam2_d = c->lambda * c->lambda;
-  const bool literal = (c->flags & PFDTD_MATSLOT_LITERAL) != 0;
-  for (int K = 0; K <= 6; ++K) {
-    for (int m = 0; m < 256; ++m) {
-      const int64_t slot = material_slot((unsigned)m, c->octave, literal);
-      const float bm = slot < (int64_t)c->n_mat * kCoefNum ? c->table[(size_t)slot] : 0.f;
-      const float position = (float)K;
-      const float beta = (float)(0.5 * (double)(bm * (6.f - position) * lam_f));
-      CoefF &f = cf[(size_t)K * 256 + m];
-      f.inv = 1.f / (1.f + beta);
-      f.omb = 1.f - beta;
-      f.tk = 2.f - position * lam2_f;
-      const double bmd = (double)bm;
-      const double positiond = (double)K;
-      const double betad = 0.5 * (bmd * (6.f - positiond) * lam_d);
-      CoefD &d = cd[(size_t)K * 256 + m];
-      d.inv = 1.f / (1.f + betad);
-      d.omb = 1.f - betad;
-      d.tk = 2.f - positiond * lam2_d;
-    }
-  }
+  build_coef_tables(c->table.data(), c->n_mat, c->octave, c->lambda,
+                    (c->flags & PFDTD_MATSLOT_LITERAL) != 0, cf.data(), cd.data());
 }
 
 int scan_domain(pfdtd_ctx *c) {

@@ -126,4 +126,138 @@ __device__ __forceinline__ C srl_node(unsigned pb, unsigned mb, C pc, C s, C pas
   return a * ((e.tk * pc + lam2 * s) - e.omb * past);
 }
 
+// ---------------------------------------------------------------------------------------
+// SRL_FORWARD update of one tile: row-segment tiles marching short z-chunks
+// ---------------------------------------------------------------------------------------
+// A tile is 256 threads = four wavefronts side by side along x, so one tile row is a
+// contiguous 1 KB (FP32) segment and every wavefront access is a full 256 B run. Each thread
+// owns RY consecutive y rows of its x column and marches them through `zc` slices, keeping
+// P[z-1] and P[z] of every row in registers (the z-neighbour queue); per slice only P[z+1],
+// P_past and the two byte arrays are new, the y-neighbours of inner rows come from registers
+// and the halo rows y0-1, y0+RY and the x-neighbours are cache hits of neighbouring threads'
+// loads. Short z-chunks keep the set of concurrently streamed slices small so the y-halo rows
+// and the slices above and below are still in L2 / Infinity Cache when they are re-read.
+// Air voxels (position byte 0x86) take their coefficients from kernel arguments; every other
+// voxel gathers its (K, material) entry from the table. Tile (bx, by, bz) covers x in
+// [256 bx, 256 bx + 256), y in [RY by, RY by + RY), z in [1 + zc bz, 1 + zc bz + zc).
+constexpr int kTX = 256;
+
+template <typename S, int RY>
+__device__ __forceinline__ void srl_tile(
+    const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
+    S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
+    typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
+    int X, int Y, int Z, int zc, int bx, int by, int bz) {
+#pragma clang fp contract(off)
+  typedef typename Fmt<S>::C C;
+  const int x = bx * kTX + (int)threadIdx.x;
+  const int ybase = by * RY;
+  if (x >= X) return;
+  const int z0 = 1 + bz * zc;
+  const int z1 = min(z0 + zc, Z - 1);
+  const int64_t sX = X;
+  const int64_t XY = (int64_t)X * (int64_t)Y;
+  const bool xin = (x > 0) && (x < X - 1);
+
+  // Row r is live when ybase + r < Y; it is updated when additionally 0 < y < Y-1 and x is
+  // interior. Rows past the grid edge are clamped to Y-1 for loads and never stored.
+  bool live[RY], upd[RY];
+  int64_t row[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int y = ybase + r;
+    live[r] = y < Y;
+    upd[r] = live[r] && xin && y > 0 && y < Y - 1;
+    row[r] = (int64_t)min(y, Y - 1) * sX + x;
+  }
+  const int64_t halo_lo = (int64_t)max(ybase - 1, 0) * sX + x;
+  const int64_t halo_hi = (int64_t)min(ybase + RY, Y - 1) * sX + x;
+
+  C pm[RY], pc[RY];
+  int64_t base = (int64_t)z0 * XY;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    pm[r] = Fmt<S>::ld(P, base - XY + row[r]);
+    pc[r] = Fmt<S>::ld(P, base + row[r]);
+  }
+  for (int z = z0; z < z1; ++z, base += XY) {
+    C pn[RY], past[RY], xp[RY], xm[RY];
+    unsigned pb[RY], mb[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int64_t i = base + row[r];
+      pn[r] = Fmt<S>::ld(P, i + XY);
+      past[r] = Fmt<S>::ld(Pp, i);
+      pb[r] = pos[i];
+      mb[r] = mat[i];
+      xp[r] = Fmt<S>::ld(P, i + 1);
+      xm[r] = Fmt<S>::ld(P, i - 1);
+    }
+    const C ylo = Fmt<S>::ld(P, base + halo_lo);
+    const C yhi = Fmt<S>::ld(P, base + halo_hi);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const C yp = (r + 1 < RY) ? pc[r + 1] : yhi;
+      const C ym = (r > 0) ? pc[r - 1] : ylo;
+      // Sum order z+1, z-1, y+1, y-1, x+1, x-1 (kernels3d.cu:516-525).
+      C s = pn[r] + pm[r];
+      s = s + yp;
+      s = s + ym;
+      s = s + xp[r];
+      s = s + xm[r];
+      typename CoefOf<C>::T e = air;
+      if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * 256u + mb[r]];
+      const C a = (C)(pb[r] >> 7) * e.inv;
+      const C ret = a * ((e.tk * pc[r] + lam2 * s) - e.omb * past[r]);
+      if (live[r]) Fmt<S>::st(Pp, base + row[r], upd[r] ? ret : (C)0);
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      pm[r] = pc[r];
+      pc[r] = pn[r];
+    }
+  }
+}
+
+// Receiver gather and source injection for one field, executed by one block (all threads
+// call it). Gather reads the field just produced (response[step] = P^{step+1},
+// kernels3d.cu:164-173); injection then applies the next step's samples to the same field, in
+// source order, which is where the reference's source loop of the next step writes them
+// (kernels3d.cu:93-104).
+template <typename S>
+__device__ __forceinline__ void inject_gather_block(S *field, const InjectArgs &a) {
+#pragma clang fp contract(off)
+  typedef typename Fmt<S>::C C;
+  if (a.gather_step >= 0) {
+    const double inv_scale = ldexp(1.0, -a.scale_exp);
+    for (int r = threadIdx.x; r < a.n_rec; r += blockDim.x) {
+      const int64_t idx = a.rec_idx[r];
+      if (idx >= 0)
+        a.resp[(int64_t)r * a.resp_steps + a.gather_step] =
+            (double)Fmt<S>::ld(field, idx) * inv_scale;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && a.inject_step >= 0) {
+    const C scale = (C)ldexp(1.0, a.scale_exp);
+    for (int s = 0; s < a.n_src; ++s) {
+      const int64_t idx = a.src_idx[s];
+      if (idx < 0) continue;
+      C smp = (C)0;
+      if (a.inject_step < a.n_samples) {
+        const int64_t off = (int64_t)s * a.n_samples + a.inject_step;
+        if (sizeof(C) == sizeof(double))
+          smp = (C)a.sig_d[off];
+        else
+          smp = (C)a.sig_f[off];
+      }
+      smp = smp * scale;
+      if (a.src_kind[s] == 0 || a.soft_overwrite)
+        Fmt<S>::st(field, idx, smp);
+      else
+        Fmt<S>::st(field, idx, Fmt<S>::ld(field, idx) + smp);
+    }
+  }
+}
+
 }  // namespace pfdtd

@@ -143,6 +143,15 @@ struct InjectArgs {
 hipError_t launch_inject_gather(int precision, void *field, const InjectArgs &a,
                                 hipStream_t stream);
 
+// Material slot of material m at `octave` (intended m*20+octave, or the reference's literal
+// m*20*octave) and the per-(K, m) coefficient tables (kCoefEntries entries each) built from a
+// [n_mat][20] admittance table.
+int64_t material_slot(unsigned m, unsigned octave, bool literal);
+// Sets the message pfdtd_last_error() returns on this thread.
+void record_error(const char *msg);
+void build_coef_tables(const float *table20, int n_mat, unsigned octave, double lambda,
+                       bool literal, CoefF *cf, CoefD *cd);
+
 hipError_t launch_classify_labels(const uint8_t *labels, uint8_t *pos, uint8_t *mat, int X,
                                   int Y, int Z, hipStream_t stream);
 hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream);

@@ -28,6 +28,18 @@ _p = C.c_void_p
 _i32, _i64, _u32, _u64, _dbl = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double
 _pi32, _pi64, _pu64, _pdbl = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_uint64), C.POINTER(C.c_double)
 
+
+
+class PfdtdJob(C.Structure):
+    """struct pfdtd_job (one serving-pool job)."""
+    _fields_ = [("labels", _p), ("X", _i32), ("Y", _i32), ("Z", _i32),
+                ("materials", _p), ("n_mat", _i32), ("n_coef", _i32), ("octave", _u32),
+                ("lambda_", _dbl), ("flags", _i32),
+                ("n_src", _i32), ("src_xyz", _p), ("src_kind", _p), ("n_samples", _i32),
+                ("signals", _p), ("n_rec", _i32), ("rec_xyz", _p), ("n_steps", _i32),
+                ("precision", _i32), ("scale_exp", _i32)]
+
+
 # name -> argtypes (all return int unless listed in _RESTYPE)
 SIGNATURES: dict[str, list] = {
     "pfdtd_last_error": [],
@@ -64,6 +76,19 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_receiver_slots": [_p, C.c_int, _p],
     "pfdtd_set_update_tiling": [_p, C.c_int, C.c_int],
     "pfdtd_set_time_levels": [_p, C.c_int, C.c_int],
+    "pfdtd_pool_create": [C.c_int, _i64, C.POINTER(_p)],
+    "pfdtd_pool_destroy": [_p],
+    "pfdtd_pool_set_options": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "pfdtd_pool_job_bytes": [C.POINTER(PfdtdJob), _pi64],
+    "pfdtd_pool_mem": [_p, _pi64, _pi64, _pi64],
+    "pfdtd_pool_submit": [_p, C.POINTER(PfdtdJob), _pi64],
+    "pfdtd_pool_advance": [_p, C.POINTER(C.c_int)],
+    "pfdtd_pool_poll": [_p, _pi64, C.c_int, C.POINTER(C.c_int)],
+    "pfdtd_pool_drain": [_p],
+    "pfdtd_pool_job_info": [_p, _i64, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "pfdtd_pool_get_responses": [_p, _i64, _p, _p, _i64],
+    "pfdtd_pool_release": [_p, _i64],
+    "pfdtd_pool_stats": [_p, _pi64, _pi64, _pdbl, C.POINTER(C.c_int)],
 }
 _RESTYPE = {"pfdtd_last_error": C.c_char_p}
 

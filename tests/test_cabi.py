@@ -39,3 +39,46 @@ def test_error_path_without_device():
     rc = lib.pfdtd_set_kernel_timing(None, 1)
     assert rc == native.E_ARG
     assert b"null" in lib.pfdtd_last_error()
+
+
+def test_job_struct_layout_matches_header(tmp_path):
+    """ctypes' pfdtd_job has the field offsets and size the C compiler gives the header's."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    fields = [f[0] for f in native.PfdtdJob._fields_]
+    cname = {"lambda_": "lambda"}
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {",
+             '  printf("size %zu\\n", sizeof(pfdtd_job));']
+    for f in fields:
+        lines.append(f'  printf("{f} %zu\\n", offsetof(pfdtd_job, {cname.get(f, f)}));')
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", "-o", str(exe), str(src)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(native.PfdtdJob)
+    for f in fields:
+        assert int(out[f]) == getattr(native.PfdtdJob, f).offset, f
+
+
+def test_pool_job_bytes_without_device():
+    """Arena footprint: two fields, three byte planes, the coefficient table, source/receiver
+    tables and the response buffer, each rounded up to 256 bytes."""
+    lib = native.lib()
+    j = native.PfdtdJob(X=10, Y=9, Z=8, n_src=3, n_samples=50, n_rec=2, n_steps=100,
+                        precision=native.F16)
+    b = ctypes.c_int64()
+    assert lib.pfdtd_pool_job_bytes(ctypes.byref(j), ctypes.byref(b)) == native.OK
+    up = lambda v: (v + 255) // 256 * 256
+    N = 10 * 9 * 8
+    expect = (2 * up(2 * N) + 3 * up(N) + up(16 * 2048) + up(8 * 3) + up(4 * 3) + up(4 * 150)
+              + up(8 * 2) + up(8 * 200))
+    assert b.value == expect
+    j.precision = native.F64
+    assert lib.pfdtd_pool_job_bytes(ctypes.byref(j), ctypes.byref(b)) == native.E_ARG
+    assert lib.pfdtd_pool_destroy(None) == native.OK
