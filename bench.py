@@ -73,7 +73,7 @@ def committed_traffic() -> float | None:
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(scene, steps: int = 60, nz: int = 66) -> dict:
+def cpu_baseline(scene, steps: int = 300, nz: int = 66) -> dict:
     """Oracle timed on a z-slab of the same room: the first nz-1 slices plus a closing shell."""
     import oracle
 
@@ -94,6 +94,25 @@ def cpu_baseline(scene, steps: int = 60, nz: int = 66) -> dict:
                       f"OpenMP over z on {threads} threads, {dt:.1f} s"}
 
 
+SERVE_SCALE = 2  # serving leg: rooms 32..256 voxels per side (the C5 stream at half size)
+SERVE_RATE = 20.0
+
+
+def serving_leg(jobs: int, rate: float) -> dict:
+    """p50/p99 impulse-response latency of the continuous-batching pool on a Poisson stream."""
+    from parallelfdtd_amd import serve
+
+    rep = serve.serve(jobs, rate, scale=SERVE_SCALE, step_scale=1, log=log)
+    return {"jobs": jobs, "rate_jobs_per_s": rate,
+            "stream": f"C5 stream at 1/{SERVE_SCALE} size: rooms {64 // SERVE_SCALE}.."
+                      f"{512 // SERVE_SCALE} per side, 0-10 boxes, 1-6 materials, 1-16 sources, "
+                      f"2000-8000 steps, FP32/FP16 1:1",
+            "p50_latency_s": round(rep["latency_s"]["p50"], 4),
+            "p99_latency_s": round(rep["latency_s"]["p99"], 4),
+            "gvox_per_s": round(rep["gvox_per_s"], 2), "makespan_s": round(rep["makespan_s"], 2),
+            "graph_builds": rep["pool"]["graph_builds"], "bundles": rep["pool"]["bundles"]}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +120,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scale", type=int, default=1, help="divide every room dimension (quick runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--serve-jobs", type=int, default=200,
+                    help="impulse-response jobs in the serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=SERVE_RATE,
+                    help="Poisson arrival rate of the serving leg, jobs/s")
     ap.add_argument("--fp32-only", action="store_true", help="only the FP32 run (profiling)")
+    ap.add_argument("--fp16-only", action="store_true",
+                    help="only the FP16-storage run (profiling); prints no bench line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 4 GiB device copy (counter calibration for rocprofv3)")
     args = ap.parse_args()
@@ -134,6 +159,13 @@ def main() -> int:
         s.load_scene(sc)
         log(f"domain classified on GPU in {time.perf_counter() - t0:.1f} s")
 
+        if args.fp16_only:
+            s.prepare_fields(N.F16, 12)
+            wall16, _ = timed_run(s, args.steps, args.warmup, N.F16)
+            k16 = kernel_avg_ms(s, min(args.steps, 100), N.F16)
+            log(f"FP16 storage: {nodes * args.steps / wall16 / 1e9:.1f} Gvox/s, "
+                f"update {k16:.3f} ms/step")
+            return 0
         wall, dev_ms = timed_run(s, args.steps, args.warmup, N.F32)
         value = nodes * args.steps / wall / 1e9
         log(f"FP32: {value:.1f} Gvox/s wall, device {nodes * args.steps / dev_ms / 1e6:.1f}")
@@ -176,6 +208,9 @@ def main() -> int:
                      "kernel_ms": round(kms, 4), "bytes_per_node": BYTES_PER_NODE[N.F32]},
         "fp16_storage": fp16,
     }
+    if args.serve_jobs > 0:
+        out["serving"] = serving_leg(args.serve_jobs, args.serve_rate)
+        log(f"serving: {out['serving']}")
     if not args.no_cpu:
         t0 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(sc)

@@ -139,10 +139,13 @@ __device__ __forceinline__ C srl_node(unsigned pb, unsigned mb, C pc, C s, C pas
 // and the slices above and below are still in L2 / Infinity Cache when they are re-read.
 // Air voxels (position byte 0x86) take their coefficients from kernel arguments; every other
 // voxel gathers its (K, material) entry from the table. Tile (bx, by, bz) covers x in
-// [256 bx, 256 bx + 256), y in [RY by, RY by + RY), z in [1 + zc bz, 1 + zc bz + zc).
+// [TW bx, TW bx + TW), y in [R by, R by + R) with R = RY * 256 / TW, z in
+// [1 + zc bz, 1 + zc bz + zc). TW = 256 (one 1 KB FP32 row segment per tile row) suits wide
+// rooms; TW = 64 (four wavefronts stacked in y) wastes fewer lanes on rooms whose width is
+// not a multiple of 256.
 constexpr int kTX = 256;
 
-template <typename S, int RY>
+template <typename S, int RY, int TW = kTX>
 __device__ __forceinline__ void srl_tile(
     const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
     S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
@@ -150,8 +153,9 @@ __device__ __forceinline__ void srl_tile(
     int X, int Y, int Z, int zc, int bx, int by, int bz) {
 #pragma clang fp contract(off)
   typedef typename Fmt<S>::C C;
-  const int x = bx * kTX + (int)threadIdx.x;
-  const int ybase = by * RY;
+  // TW threads along x; 256 / TW row bands of RY rows each stacked in y
+  const int x = bx * TW + (int)(threadIdx.x % TW);
+  const int ybase = (by * (kTX / TW) + (int)(threadIdx.x / TW)) * RY;
   if (x >= X) return;
   const int z0 = 1 + bz * zc;
   const int z1 = min(z0 + zc, Z - 1);

@@ -10,6 +10,8 @@
 // evaluation of the reference expression.
 #include "pfdtd_device.h"
 
+#include <algorithm>
+
 namespace pfdtd {
 
 // ---------------------------------------------------------------------------------------
@@ -302,48 +304,81 @@ hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStre
 }
 
 // Counts air/boundary voxels (calcBoundaries, cudaMesh.cu:500-514) and checks the
-// preconditions the update relies on. One block per (y, z) row segment of 256 x.
+// preconditions the update relies on. Blocks stride over (y, z) rows, threads over x; each
+// thread keeps private counters and a 64-bit set of the material bytes below 64 it saw, the
+// wavefront reduces them with cross-lane shuffles, and only one atomic per wavefront and
+// counter reaches LDS (one per block reaches global memory).
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_or(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+  return v;
+}
+
 __global__ __launch_bounds__(256) void scan_domain_kernel(const uint8_t *__restrict__ pos,
                                                           const uint8_t *__restrict__ mat, int X,
                                                           int Y, int Z, DomainScan *out) {
-  __shared__ unsigned int s_air, s_bnd, s_shell, s_badk;
+  __shared__ unsigned long long s_cnt[4];
   __shared__ unsigned int s_mat[8];
   if (threadIdx.x < 8) s_mat[threadIdx.x] = 0;
-  if (threadIdx.x == 0) {
-    s_air = 0;
-    s_bnd = 0;
-    s_shell = 0;
-    s_badk = 0;
+  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long air = 0, bnd = 0, shell = 0, badk = 0, mlo = 0;
+  const int64_t rows = (int64_t)Y * Z;
+  for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+    const int y = (int)(row % Y);
+    const int z = (int)(row / Y);
+    const bool edge_row = (y == 0 || y == Y - 1 || z == 0 || z == Z - 1);
+    const uint8_t *prow = pos + row * X;
+    const uint8_t *mrow = mat + row * X;
+    for (int x = threadIdx.x; x < X; x += 256) {
+      const unsigned p = prow[x];
+      const unsigned m = mrow[x];
+      air += (p == 0x86u);
+      bnd += (p != 0u && p != 0x86u);
+      if (p >> 7) {
+        badk += ((p & 0x7Fu) > 6u);
+        shell += (edge_row || x == 0 || x == X - 1);
+      }
+      if (m < 64u)
+        mlo |= 1ull << m;
+      else
+        atomicOr(&s_mat[m >> 5], 1u << (m & 31u));
+    }
+  }
+  air = wave_sum(air);
+  bnd = wave_sum(bnd);
+  shell = wave_sum(shell);
+  badk = wave_sum(badk);
+  mlo = wave_or(mlo);
+  if ((threadIdx.x & 63) == 0) {
+    if (air) atomicAdd(&s_cnt[0], air);
+    if (bnd) atomicAdd(&s_cnt[1], bnd);
+    if (shell) atomicAdd(&s_cnt[2], shell);
+    if (badk) atomicAdd(&s_cnt[3], badk);
+    if (mlo & 0xFFFFFFFFull) atomicOr(&s_mat[0], (unsigned)(mlo & 0xFFFFFFFFull));
+    if (mlo >> 32) atomicOr(&s_mat[1], (unsigned)(mlo >> 32));
   }
   __syncthreads();
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  const int y = blockIdx.y;
-  const int z = blockIdx.z;
-  if (x < X) {
-    const int64_t i = ((int64_t)z * Y + y) * X + x;
-    const unsigned p = pos[i];
-    const unsigned m = mat[i];
-    if (p == 0x86u) atomicAdd(&s_air, 1u);
-    if (p != 0u && p != 0x86u) atomicAdd(&s_bnd, 1u);
-    const bool shell = (x == 0 || x == X - 1 || y == 0 || y == Y - 1 || z == 0 || z == Z - 1);
-    if (shell && (p >> 7)) atomicAdd(&s_shell, 1u);
-    if ((p >> 7) && (p & 0x7Fu) > 6u) atomicAdd(&s_badk, 1u);
-    atomicOr(&s_mat[m >> 5], 1u << (m & 31u));
-  }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_air) atomicAdd(&out->air, (unsigned long long)s_air);
-    if (s_bnd) atomicAdd(&out->boundary, (unsigned long long)s_bnd);
-    if (s_shell) atomicAdd(&out->shell_inside, s_shell);
-    if (s_badk) atomicAdd(&out->bad_k, s_badk);
+    if (s_cnt[0]) atomicAdd(&out->air, s_cnt[0]);
+    if (s_cnt[1]) atomicAdd(&out->boundary, s_cnt[1]);
+    if (s_cnt[2]) atomicAdd(&out->shell_inside, (unsigned)min(s_cnt[2], 0xFFFFFFFFull));
+    if (s_cnt[3]) atomicAdd(&out->bad_k, (unsigned)min(s_cnt[3], 0xFFFFFFFFull));
   }
   if (threadIdx.x < 8 && s_mat[threadIdx.x]) atomicOr(&out->mat_present[threadIdx.x], s_mat[threadIdx.x]);
 }
 
 hipError_t launch_scan_domain(const uint8_t *pos, const uint8_t *mat, int X, int Y, int Z,
                               DomainScan *d_out, hipStream_t stream) {
-  const dim3 grid((X + 255) / 256, Y, Z), block(256);
-  hipLaunchKernelGGL(scan_domain_kernel, grid, block, 0, stream, pos, mat, X, Y, Z, d_out);
+  const int64_t rows = (int64_t)Y * Z;
+  const int blocks = (int)std::min<int64_t>(rows, 8192);
+  hipLaunchKernelGGL(scan_domain_kernel, dim3(blocks), dim3(256), 0, stream, pos, mat, X, Y, Z,
+                     d_out);
   return hipGetLastError();
 }
 

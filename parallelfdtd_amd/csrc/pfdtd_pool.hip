@@ -33,6 +33,7 @@
 namespace pfdtd {
 
 constexpr int kPoolMaxJobs = 64;
+constexpr int kPoolTW = 64;  // tile width: rooms of any width waste fewer than 64 lanes per row
 
 struct PoolJobDev {
   const uint8_t *pos;
@@ -90,8 +91,8 @@ __global__ __launch_bounds__(kTX) void pool_update_kernel(const PoolJobDev *__re
   const int bz = r / J.gy;
   const S *P = (const S *)(st.parity ? J.B : J.A);
   S *Pp = (S *)(st.parity ? J.A : J.B);
-  srl_tile<S, RY>(J.pos, J.mat, P, Pp, J.coef, J.air, J.air_uniform, J.lam2, J.X, J.Y, J.Z, zc,
-                  bx, by, bz);
+  srl_tile<S, RY, kPoolTW>(J.pos, J.mat, P, Pp, J.coef, J.air, J.air_uniform, J.lam2, J.X, J.Y,
+                           J.Z, zc, bx, by, bz);
 }
 
 template <typename S>
@@ -625,10 +626,15 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   d.X = X;
   d.Y = Y;
   d.Z = Z;
-  const UpdateGeom g = plan_update(X, Y, Z, p->ry, p->zc);
-  d.gx = (int)g.grid.x;
-  d.gy = (int)g.grid.y;
-  const int64_t tiles = (int64_t)g.grid.x * g.grid.y * g.grid.z;
+  const int rows_per_tile = p->ry * (kTX / kPoolTW);
+  d.gx = (X + kPoolTW - 1) / kPoolTW;
+  d.gy = (Y + rows_per_tile - 1) / rows_per_tile;
+  const int gz = (Z - 2 + p->zc - 1) / p->zc;
+  const int64_t tiles = (int64_t)d.gx * d.gy * gz;
+  if (tiles > (int64_t)1 << 30) {
+    arena_free(p, off, L.total);
+    return pfail(PFDTD_E_ARG, "job grid too large for one launch");
+  }
   d.rec_idx = (const int64_t *)(base + L.rec_idx);
   d.resp = (double *)(base + L.resp);
   d.src_idx = (const int64_t *)(base + L.src_idx);

@@ -115,3 +115,21 @@ def test_levels_agree_on_c1(gpu):
             s.run(sc.n_steps, N.F32)
             out.append(s.responses(np.float32))
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("prec", [N.F32, N.F16])
+def test_two_level_sources_on_tile_edges(gpu, prec):
+    """Sources and receivers on the columns and rows where 64 x 16 tiles meet (their one-voxel
+    rings overlap there), including the last partial tile of each axis."""
+    sc = rooms.scene_random_small(86, dims=(131, 69, 23), n_mat=3, n_boxes=3, n_src=1, n_rec=1,
+                                  n_steps=41)
+    xs = [1, 62, 63, 64, 65, 127, 128, 129]
+    ys = [1, 15, 16, 17, 31, 32, 47, 48, 64, 67]
+    pts = np.array([[xs[i % len(xs)], ys[(3 * i) % len(ys)], 2 + (5 * i) % 19] for i in range(40)],
+                   np.int32)
+    rng = np.random.default_rng(7)
+    sc.src_xyz = pts[:24].copy()
+    sc.src_kind = np.array([i % 2 for i in range(24)], np.int32)
+    sc.signals = rng.uniform(-1, 1, (24, sc.n_steps))
+    sc.rec_xyz = pts[::-1].copy()
+    assert_same(*run_both(sc, prec, zchunk=7))
