@@ -117,6 +117,19 @@ int pfdtd_set_sources(pfdtd_ctx *ctx, int n_src, const int32_t *xyz, const int32
  * SimulationParameters.cpp:213-224). Receivers outside the grid record 0 (kernels3d.cu:168). */
 int pfdtd_set_receivers(pfdtd_ctx *ctx, int n_rec, const int32_t *xyz);
 
+/* ---- update type -------------------------------------------------------------------- */
+/* enum UpdateType (src/base/SimulationParameters.h:39): 0 SRL_FORWARD (forward boundaries,
+ * fdtd3dStdMaterials, kernels3d.cu:485-529; the default), 1 SHARED (the same update: the
+ * reference's sliced kernel fdtd3dSliced, kernels3d.cu:531-607, implements this formula; its
+ * unmasked position byte and LDS race are not reproduced), 2 SRL (centred boundaries,
+ * fdtd3dStdKowalczykMaterials, kernels3d.cu:609-665). The domain's position bytes encode the
+ * boundary formulation (toBilbao vs toKowalczyk, cudaMesh.cu:66-74): set the type before the
+ * domain; switching between the forward and centred families discards a classified domain.
+ * Labels are classified for the centred scheme as toKowalczyk encodes voxelizer codes (inside
+ * voxels with no air neighbour on some axis have no centred stencil: PFDTD_E_DOMAIN at run).
+ * Type 2 runs the single-level, unbatched update. */
+int pfdtd_set_update_type(pfdtd_ctx *ctx, int type);
+
 /* ---- source batching ---------------------------------------------------------------- */
 /* Advance `slots` independent source configurations on the same domain in one launch per
  * step (power of two, 1..256; 1 = unbatched, the default). Fields then hold `slots` values per

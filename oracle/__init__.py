@@ -29,6 +29,10 @@ def lib() -> C.CDLL:
         L.orc_to_bilbao.argtypes = [C.c_uint8, _p]
         L.orc_classify_labels.restype = None
         L.orc_classify_labels.argtypes = [_p, _i64, _i64, _i64, _p, _p, _p]
+        L.orc_to_kowalczyk.restype = C.c_uint8
+        L.orc_to_kowalczyk.argtypes = [C.c_uint8, _p]
+        L.orc_classify_labels_kow.restype = C.c_uint64
+        L.orc_classify_labels_kow.argtypes = [_p, _i64, _i64, _i64, _p, _p, _p]
         L.orc_param_vector_f.restype = None
         L.orc_param_vector_f.argtypes = [C.c_double, C.c_uint, _p]
         L.orc_param_vector_d.restype = None
@@ -90,7 +94,19 @@ def classify(labels: np.ndarray):
     return pos, mat, counts
 
 
+def classify_kow(labels: np.ndarray):
+    """Centred-scheme (update type SRL) (pos, mat, counts[3]) for a (Z, Y, X) label grid."""
+    labels = np.ascontiguousarray(labels, np.uint8)
+    Z, Y, X = labels.shape
+    pos = np.empty_like(labels)
+    mat = np.empty_like(labels)
+    counts = np.zeros(3, np.uint64)
+    lib().orc_classify_labels_kow(_ptr(labels), X, Y, Z, _ptr(pos), _ptr(mat), _ptr(counts))
+    return pos, mat, counts
+
+
 STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+SCHEME_KOWALCZYK = 4  # bit of `literal`: update type SRL (centred boundary)
 
 
 def run(pos, mat, table, src_xyz, src_kind, signals, rec_xyz, n_steps, *, precision="f32",

@@ -81,6 +81,89 @@ void orc_classify_labels(const uint8_t *label, int64_t X, int64_t Y, int64_t Z, 
   if (counts) memcpy(counts, c, sizeof(c));
 }
 
+/* toKowalczyk, src/kernels/cudaMesh.cu:363-480: voxelizer code -> centred-scheme position
+ * byte (0x80 inside | DIR_X 0x01 | DIR_Y 0x02 | DIR_Z 0x04 | SIGN_X 0x10 | SIGN_Y 0x20 |
+ * SIGN_Z 0x40, cudaMesh.h:35-44). Restated from the code comments' air-direction lists
+ * ("Left/Right" = -x/+x, "In/Out" = -y/+y, "Down/Up" = -z/+z): an axis with exactly one air
+ * neighbour gets its DIR bit, and its SIGN bit selects the present neighbour (+x, +y, -z). */
+uint8_t orc_to_kowalczyk(uint8_t k, uint8_t *mat_inout) {
+  /* air-neighbour sets per code: bit0 -x, bit1 +x, bit2 -y, bit3 +y, bit4 -z, bit5 +z */
+  static const uint8_t present[28] = {
+      0,
+      0x01 | 0x04 | 0x10, 0x02 | 0x04 | 0x10, 0x01 | 0x08 | 0x10, 0x02 | 0x08 | 0x10, /* 1-4 down */
+      0x01 | 0x04 | 0x20, 0x02 | 0x04 | 0x20, 0x01 | 0x08 | 0x20, 0x02 | 0x08 | 0x20, /* 5-8 up */
+      0x10 | 0x03 | 0x04, 0x10 | 0x03 | 0x08, 0x10 | 0x01 | 0x0C, 0x10 | 0x02 | 0x0C, /* 9-12 */
+      0x20 | 0x03 | 0x04, 0x20 | 0x03 | 0x08, 0x20 | 0x01 | 0x0C, 0x20 | 0x02 | 0x0C, /* 13-16 */
+      0x30 | 0x01 | 0x04, 0x30 | 0x02 | 0x04, 0x30 | 0x01 | 0x08, 0x30 | 0x02 | 0x08, /* 17-20 */
+      0x03 | 0x0C | 0x10, 0x03 | 0x08 | 0x30, 0x03 | 0x04 | 0x30, 0x02 | 0x0C | 0x30, /* 21-24 */
+      0x01 | 0x0C | 0x30, 0x03 | 0x0C | 0x20, 0x3F};                                 /* 25-27 */
+  if (k == 0) {
+    if (mat_inout) *mat_inout = 0;
+    return 0;
+  }
+  if (k > 27) return k;
+  const uint8_t a = present[k];
+  uint8_t pb = 0x80;
+  if ((a & 0x03) == 0x01) pb |= 0x01;        /* only -x present: mirror -x */
+  if ((a & 0x03) == 0x02) pb |= 0x01 | 0x10; /* only +x present */
+  if ((a & 0x0C) == 0x04) pb |= 0x02;
+  if ((a & 0x0C) == 0x08) pb |= 0x02 | 0x20;
+  if ((a & 0x30) == 0x10) pb |= 0x04 | 0x40; /* only -z present: SIGN_Z ("down") */
+  if ((a & 0x30) == 0x20) pb |= 0x04;
+  return pb;
+}
+
+/* Centred-scheme classification of a label grid (same conventions as orc_classify_labels):
+ * inside voxels get 0x80 | DIR/SIGN bits as orc_to_kowalczyk would give for their air-neighbour
+ * set; an inside voxel with no air neighbour on some axis has no centred stencil (the voxelizer
+ * never emits one) and is counted in counts[2]. counts[0] = air (0x80), counts[1] = boundary,
+ * following calcBoundaries with air value 0x80 (cudaMesh.cu:176-186). Returns counts[2]. */
+uint64_t orc_classify_labels_kow(const uint8_t *label, int64_t X, int64_t Y, int64_t Z,
+                                 uint8_t *pos, uint8_t *mat, uint64_t counts[3]) {
+  const int64_t XY = X * Y;
+  uint64_t c[3] = {0, 0, 0};
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t y = 0; y < Y; ++y)
+      for (int64_t x = 0; x < X; ++x) {
+        const int64_t i = z * XY + y * X + x;
+        if (label[i] != 0) {
+          pos[i] = 0;
+          mat[i] = 0;
+          continue;
+        }
+        const int inb[6] = {x > 0, x < X - 1, y > 0, y < Y - 1, z > 0, z < Z - 1};
+        const int64_t off[6] = {-1, 1, -X, X, -XY, XY};
+        unsigned air = 0, m = 0;
+        int found = 0;
+        for (int d = 0; d < 6; ++d) {
+          const unsigned lab = inb[d] ? label[i + off[d]] : 1u;
+          if (lab == 0)
+            air |= 1u << d;
+          else if (!found) {
+            found = 1;
+            m = inb[d] ? lab - 1u : 0u;
+          }
+        }
+        uint8_t pb = 0x80;
+        int bad = 0;
+        for (int a = 0; a < 3; ++a) {
+          const unsigned two = (air >> (2 * a)) & 3u;
+          if (two == 0) bad = 1;
+          if (two == 1) pb |= (uint8_t)((1u << a) | (a == 2 ? 0x40u : 0u));
+          if (two == 2) pb |= (uint8_t)((1u << a) | (a == 2 ? 0u : (0x10u << a)));
+        }
+        pos[i] = pb;
+        mat[i] = (uint8_t)(air == 0x3F ? 0 : m);
+        if (bad) c[2]++;
+        if (pb == 0x80)
+          c[0]++;
+        else
+          c[1]++;
+      }
+  if (counts) memcpy(counts, c, sizeof(c));
+  return c[2];
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Parameters, materials, coordinates, source signals                                   */
 /* ------------------------------------------------------------------------------------ */
@@ -331,7 +414,7 @@ static int check_common(const uint8_t *pos, const uint8_t *mat, int64_t X, int64
   const int64_t N = X * Y * Z;
   for (int64_t i = 0; i < N; ++i) seen[mat[i]] = 1;
   for (int m = 0; m < 256; ++m)
-    if (seen[m] && orc_material_slot((unsigned)m, octave, literal & 1) >= (int64_t)n_mat * ORC_COEF_NUM)
+    if (seen[m] && orc_material_slot((unsigned)m, octave, (literal & ORC_SCHEME_KOWALCZYK) ? 0 : (literal & 1)) >= (int64_t)n_mat * ORC_COEF_NUM)
       return -1;
   if (!shell_is_outside(pos, X, Y, Z)) return -2;
   return 0;
@@ -417,6 +500,97 @@ static void sweep_d(const uint8_t *pos, const uint8_t *mat, const double *P, dou
   }
 }
 
+/* One fdtd3dStdKowalczykMaterials<float> sweep (kernels3d.cu:609-665, update type SRL, the
+ * centred boundary): the missing neighbour of an axis with a DIR bit is replaced by the present
+ * one that its SIGN bit selects (p_x[0] = x-1, p_x[1] = x+1, p_y likewise, p_z[0] = z+1,
+ * p_z[1] = z-1); beta = table[mat*20+octave] * lambda * (number of DIR bits). */
+static void sweep_kow_f(const uint8_t *pos, const uint8_t *mat, const float *P, float *Pp,
+                        const float *beta_of_mat, float lam, float lam2, int64_t X, int64_t Y,
+                        int64_t Z, int storage, int threads) {
+  const int64_t XY = X * Y;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 1; z <= Z - 2; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      const int64_t row = z * XY + y * X;
+      if (y == 0 || y == Y - 1) {
+        for (int64_t x = 0; x < X; ++x) Pp[row + x] = 0.f;
+        continue;
+      }
+      Pp[row] = 0.f;
+      Pp[row + X - 1] = 0.f;
+      for (int64_t x = 1; x <= X - 2; ++x) {
+        const int64_t i = row + x;
+        const uint8_t pb = pos[i];
+        const float switch_bit = (float)(pb >> ORC_INSIDE_SHIFT);
+        const float dir_x = (float)(pb & 0x01);
+        const float dir_y = (float)((pb & 0x02) >> 1);
+        const float dir_z = (float)((pb & 0x04) >> 2);
+        const float beta = beta_of_mat[mat[i]] * lam * (dir_x + dir_y + dir_z);
+        float p_z[2], p_y[2], p_x[2];
+        p_z[1] = P[i - XY];
+        p_z[0] = P[i + XY];
+        p_y[0] = P[i - X];
+        p_y[1] = P[i + X];
+        p_x[0] = P[i - 1];
+        p_x[1] = P[i + 1];
+        const float p = P[i];
+        const float p_past = Pp[i];
+        const int sx = (pb & 0x10) >> 4, sy = (pb & 0x20) >> 5, sz = (pb & 0x40) >> 6;
+        const float s_b = p_x[sx] * dir_x + p_y[sy] * dir_y + p_z[sz] * dir_z;
+        const float S = (p_x[0] + p_x[1] + p_y[0] + p_y[1] + p_z[0] + p_z[1] + s_b) * lam2;
+        const float ret =
+            (S - (2 - 6 * lam2) * p + (beta - 1.f) * p_past) * switch_bit * (1.f / (1.f + beta));
+        Pp[i] = store_round(ret, storage);
+      }
+    }
+  }
+}
+
+/* fdtd3dStdKowalczykMaterials<double> sweep (kernels3d.cu:609-665). */
+static void sweep_kow_d(const uint8_t *pos, const uint8_t *mat, const double *P, double *Pp,
+                        const double *beta_of_mat, double lam, double lam2, int64_t X, int64_t Y,
+                        int64_t Z, int threads) {
+  const int64_t XY = X * Y;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 1; z <= Z - 2; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      const int64_t row = z * XY + y * X;
+      if (y == 0 || y == Y - 1) {
+        for (int64_t x = 0; x < X; ++x) Pp[row + x] = 0.0;
+        continue;
+      }
+      Pp[row] = 0.0;
+      Pp[row + X - 1] = 0.0;
+      for (int64_t x = 1; x <= X - 2; ++x) {
+        const int64_t i = row + x;
+        const uint8_t pb = pos[i];
+        const double switch_bit = (double)(pb >> ORC_INSIDE_SHIFT);
+        const double dir_x = (double)(pb & 0x01);
+        const double dir_y = (double)((pb & 0x02) >> 1);
+        const double dir_z = (double)((pb & 0x04) >> 2);
+        const double beta = beta_of_mat[mat[i]] * lam * (dir_x + dir_y + dir_z);
+        double p_z[2], p_y[2], p_x[2];
+        p_z[1] = P[i - XY];
+        p_z[0] = P[i + XY];
+        p_y[0] = P[i - X];
+        p_y[1] = P[i + X];
+        p_x[0] = P[i - 1];
+        p_x[1] = P[i + 1];
+        const double p = P[i];
+        const double p_past = Pp[i];
+        const int sx = (pb & 0x10) >> 4, sy = (pb & 0x20) >> 5, sz = (pb & 0x40) >> 6;
+        const double s_b = p_x[sx] * dir_x + p_y[sy] * dir_y + p_z[sz] * dir_z;
+        const double S = (p_x[0] + p_x[1] + p_y[0] + p_y[1] + p_z[0] + p_z[1] + s_b) * lam2;
+        Pp[i] = (S - (2 - 6 * lam2) * p + (beta - 1.f) * p_past) * switch_bit * (1.f / (1.f + beta));
+      }
+    }
+  }
+}
+
 int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
               const float *mat_table, int n_mat, unsigned octave, double lambda,
               const int32_t *src_xyz, const int32_t *src_type, const float *signals, int n_src,
@@ -430,7 +604,7 @@ int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
   orc_param_vector_f(lambda, octave, params);
   float beta_of_mat[256];
   for (int m = 0; m < 256; ++m) {
-    const int64_t slot = orc_material_slot((unsigned)m, octave, literal & 1);
+    const int64_t slot = orc_material_slot((unsigned)m, octave, (literal & ORC_SCHEME_KOWALCZYK) ? 0 : (literal & 1));
     beta_of_mat[m] = slot < (int64_t)n_mat * ORC_COEF_NUM ? mat_table[slot] : 0.f;
   }
   const int64_t N = X * Y * Z;
@@ -457,7 +631,10 @@ int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
       else
         P[i] = store_round(P[i] + smp, storage);
     }
-    sweep_f(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, storage, threads);
+    if (literal & ORC_SCHEME_KOWALCZYK)
+      sweep_kow_f(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, storage, threads);
+    else
+      sweep_f(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, storage, threads);
     float *t = P; /* flipPressurePointers, cudaMesh.h:766-779 */
     P = Pp;
     Pp = t;
@@ -488,7 +665,7 @@ int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
   orc_param_vector_d(lambda, octave, params);
   double beta_of_mat[256];
   for (int m = 0; m < 256; ++m) {
-    const int64_t slot = orc_material_slot((unsigned)m, octave, literal & 1);
+    const int64_t slot = orc_material_slot((unsigned)m, octave, (literal & ORC_SCHEME_KOWALCZYK) ? 0 : (literal & 1));
     beta_of_mat[m] = slot < (int64_t)n_mat * ORC_COEF_NUM ? mat_table[slot] : 0.0;
   }
   const int64_t N = X * Y * Z;
@@ -512,7 +689,10 @@ int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
       else
         P[i] = P[i] + smp;
     }
-    sweep_d(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, threads);
+    if (literal & ORC_SCHEME_KOWALCZYK)
+      sweep_kow_d(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, threads);
+    else
+      sweep_d(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, threads);
     double *t = P;
     P = Pp;
     Pp = t;

@@ -38,6 +38,10 @@ extern "C" {
 #define ORC_STORE_F16 1
 #define ORC_STORE_BF16 2
 
+/* Bit 2 of the `literal` argument of orc_run_*: update type SRL (centred Kowalczyk boundary,
+ * enum UpdateType, src/base/SimulationParameters.h:39) instead of SRL_FORWARD. */
+#define ORC_SCHEME_KOWALCZYK 4
+
 int orc_version(void);
 
 /* Voxelizer code k (0..27) -> Bilbao position byte, material byte zeroed for k==0.
@@ -55,6 +59,18 @@ uint8_t orc_to_bilbao(uint8_t k, uint8_t *mat_inout);
  * Pinned by: closed-form box-room counts, brute-force neighbour counting in numpy. */
 void orc_classify_labels(const uint8_t *label, int64_t X, int64_t Y, int64_t Z,
                          uint8_t *pos, uint8_t *mat, uint64_t counts[9]);
+
+/* Voxelizer code k (0..27) -> centred-scheme (update type SRL) position byte: toKowalczyk,
+ * src/kernels/cudaMesh.cu:363-480, restated from each code's list of air directions.
+ * Pinned by: the exhaustive byte table of toKowalczyk transcribed as data in the tests, and
+ * consistency with orc_classify_labels_kow on every 3x3x3 neighbourhood. */
+uint8_t orc_to_kowalczyk(uint8_t k, uint8_t *mat_inout);
+
+/* Label grid -> centred-scheme position/material bytes; returns the number of inside voxels
+ * with no air neighbour on some axis (no centred stencil). counts: air (0x80), boundary,
+ * degenerate. Pinned by: orc_to_kowalczyk on codes derived from each voxel's neighbourhood. */
+uint64_t orc_classify_labels_kow(const uint8_t *label, int64_t X, int64_t Y, int64_t Z,
+                                 uint8_t *pos, uint8_t *mat, uint64_t counts[3]);
 
 /* Parameter vector [lambda, lambda^2, 1/3, octave]:
  * SimulationParameters::getParameterPtr(+Double), src/base/SimulationParameters.cpp:379-396.
@@ -113,7 +129,8 @@ int64_t orc_material_slot(unsigned m, unsigned octave, int literal);
  * transparent-filtered where applicable). Sources/receivers outside the grid are skipped
  * (receivers then stay 0, as kernels3d.cu:168).
  * literal: bit0 = Q1 material slot (see orc_material_slot), bit1 = Q2 soft source overwrites
- * (cudaMesh.h:362-366) instead of adding.
+ * (cudaMesh.h:362-366) instead of adding, bit2 (ORC_SCHEME_KOWALCZYK) = centred update
+ * fdtd3dStdKowalczykMaterials (kernels3d.cu:609-665; its material slot is always m*20+octave).
  * responses: [n_rec][n_steps], response[r][n] = P^{n+1}[rec] (post-flip read, kernels3d.cu:164-173).
  * P_out / Pp_out (optional): the final current and past fields after the last flip.
  * storage (f32 entry only): ORC_STORE_* with field values stored scaled by 2^scale_exp,

@@ -341,7 +341,11 @@ void App::initializeGeometryFromFile(const std::string &path) {
 void App::initializeVoxels(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y,
                            int64_t Z) {
   ensureContext();
+  // position bytes are in the encoding of the current update type (toBilbao / toKowalczyk)
+  check(pfdtd_set_update_type(ctx_, (int)m_parameters.getUpdateType()), "initializeVoxels");
   check(pfdtd_set_domain(ctx_, pos, mat, X, Y, Z), "initializeVoxels");
+  labels_.clear();
+  domain_type_ = (int)m_parameters.getUpdateType();
   uint64_t air = 0, bnd = 0;
   check(pfdtd_get_counts(ctx_, &air, &bnd), "initializeVoxels");
   m_mesh.set(X, Y, Z, air, bnd);
@@ -350,7 +354,15 @@ void App::initializeVoxels(const uint8_t *pos, const uint8_t *mat, int64_t X, in
 
 void App::initializeVoxelLabels(const uint8_t *labels, int64_t X, int64_t Y, int64_t Z) {
   ensureContext();
+  check(pfdtd_set_update_type(ctx_, (int)m_parameters.getUpdateType()), "initializeVoxelLabels");
   check(pfdtd_set_domain_labels(ctx_, labels, X, Y, Z), "initializeVoxelLabels");
+  // kept so that a later setUpdateType can reclassify (the reference classifies for the
+  // update type when the mesh is initialised, cudaMesh.cu:66-74)
+  labels_.assign(labels, labels + (size_t)(X * Y * Z));
+  labels_dims_[0] = X;
+  labels_dims_[1] = Y;
+  labels_dims_[2] = Z;
+  domain_type_ = (int)m_parameters.getUpdateType();
   uint64_t air = 0, bnd = 0;
   check(pfdtd_get_counts(ctx_, &air, &bnd), "initializeVoxelLabels");
   m_mesh.set(X, Y, Z, air, bnd);
@@ -366,6 +378,26 @@ void App::initializeMesh() {
              "no voxel domain: call initializeVoxels/initializeVoxelLabels (triangle "
              "voxelization is not part of this library)");
     throw(-1);
+  }
+  const int type = (int)m_parameters.getUpdateType();
+  if ((type == SRL) != (domain_type_ == SRL)) {
+    // the boundary formulation changed since the domain was classified
+    if (labels_.empty()) {
+      log_line("initializeMesh",
+               "update type changed after initializeVoxels; pass position bytes of the new "
+               "type's encoding");
+      throw(-1);
+    }
+    check(pfdtd_set_update_type(ctx_, type), "initializeMesh");
+    check(pfdtd_set_domain_labels(ctx_, labels_.data(), labels_dims_[0], labels_dims_[1],
+                                  labels_dims_[2]),
+          "initializeMesh");
+    uint64_t air = 0, bnd = 0;
+    check(pfdtd_get_counts(ctx_, &air, &bnd), "initializeMesh");
+    m_mesh.set(labels_dims_[0], labels_dims_[1], labels_dims_[2], air, bnd);
+    domain_type_ = type;
+  } else {
+    check(pfdtd_set_update_type(ctx_, type), "initializeMesh");
   }
   const std::vector<float> table = m_materials.getMaterialCoefficients();
   if (table.empty()) {

@@ -332,6 +332,9 @@ class App {
   pfdtd_ctx *ctx_ = nullptr;
   int device_ = 0;
   bool has_domain_ = false;
+  std::vector<uint8_t> labels_;  // label grid of initializeVoxelLabels (reclassification)
+  int64_t labels_dims_[3] = {0, 0, 0};
+  int domain_type_ = 0;          // update type the resident domain was classified for
   int force_partition_to_ = -1;
   int quirk_flags_ = 0;
   float capture_db_ = 60.f;

@@ -87,6 +87,9 @@ struct pfdtd_ctx {
   CoefF air_f{};          // K = 6 row entry shared by all air voxels
   CoefD air_d{};
   int air_uniform = 0;    // 1 when every material gives the same K = 6 entry (both tables)
+  int update_type = 0;    // enum UpdateType: 0 SRL_FORWARD, 1 SHARED, 2 SRL (centred)
+  int scheme = 0;         // boundary formulation: 0 forward (types 0, 1), 2 centred (type 2)
+  unsigned degenerate = 0;  // inside voxels without a centred stencil (label classification)
   int ry = 2;             // y rows per thread of the update kernel
   int zc = 8;             // slices per z-chunk of the update kernel
   int levels = 0;         // time levels per sweep: 1, 2, or 0 = by precision
@@ -180,6 +183,40 @@ void build_coef_tables(const float *table20, int n_mat, unsigned octave, double 
   }
 }
 
+// Centred-boundary coefficients, in the operation order of fdtd3dStdKowalczykMaterials
+// (kernels3d.cu:626-664): beta = table[m*20+octave] * lambda * D (D = number of DIR bits),
+// entries {1/(1+beta), beta-1, 2-6 lambda^2}.
+void build_centred_tables(const float *table20, int n_mat, unsigned octave, double lambda,
+                          CoefF *cf, CoefD *cd) {
+#pragma clang fp contract(off)
+  const float lam_f = (float)lambda;
+  const float lam2_f = (float)(lambda * lambda);
+  const double lam_d = lambda;
+  const double lam2_d = lambda * lambda;
+  const float c_f = 2 - 6 * lam2_f;
+  const double c_d = 2 - 6 * lam2_d;
+  for (int i = 0; i < kCoefEntries; ++i) {
+    cf[i] = CoefF{0.f, 0.f, 0.f, 0.f};
+    cd[i] = CoefD{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int D = 0; D <= 3; ++D) {
+    for (int m = 0; m < 256; ++m) {
+      const int64_t slot = material_slot((unsigned)m, octave, false);
+      const float bm = slot < (int64_t)n_mat * kCoefNum ? table20[(size_t)slot] : 0.f;
+      const float beta = bm * lam_f * (float)D;
+      CoefF &f = cf[(size_t)D * 256 + m];
+      f.inv = 1.f / (1.f + beta);
+      f.omb = beta - 1.f;
+      f.tk = c_f;
+      const double betad = (double)bm * lam_d * (double)D;
+      CoefD &d = cd[(size_t)D * 256 + m];
+      d.inv = 1.f / (1.f + betad);
+      d.omb = betad - 1.f;
+      d.tk = c_d;
+    }
+  }
+}
+
 }  // namespace pfdtd
 
 namespace {
@@ -221,14 +258,17 @@ int upload_xyz(const pfdtd_ctx *c, const std::vector<int32_t> &xyz, int32_t *dst
 void build_coefficients(const pfdtd_ctx *c, std::vector<CoefF> &cf, std::vector<CoefD> &cd) {
   cf.assign(kCoefEntries, CoefF{0.f, 0.f, 0.f, 0.f});
   cd.assign(kCoefEntries, CoefD{0.0, 0.0, 0.0, 0.0});
-  build_coef_tables(c->table.data(), c->n_mat, c->octave, c->lambda,
-                    (c->flags & PFDTD_MATSLOT_LITERAL) != 0, cf.data(), cd.data());
+  if (c->scheme == 2)
+    build_centred_tables(c->table.data(), c->n_mat, c->octave, c->lambda, cf.data(), cd.data());
+  else
+    build_coef_tables(c->table.data(), c->n_mat, c->octave, c->lambda,
+                      (c->flags & PFDTD_MATSLOT_LITERAL) != 0, cf.data(), cd.data());
 }
 
 int scan_domain(pfdtd_ctx *c) {
   HIP_TRY(hipMemsetAsync(c->d_scan, 0, sizeof(DomainScan), c->stream));
-  HIP_TRY(launch_scan_domain(c->d_pos, c->d_mat, (int)c->X, (int)c->Y, (int)c->Z, c->d_scan,
-                             c->stream));
+  HIP_TRY(launch_scan_domain(c->d_pos, c->d_mat, (int)c->X, (int)c->Y, (int)c->Z, c->scheme,
+                             c->d_scan, c->stream));
   HIP_TRY(hipMemcpyAsync(&c->scan, c->d_scan, sizeof(DomainScan), hipMemcpyDeviceToHost,
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -321,7 +361,13 @@ int check_runnable(pfdtd_ctx *c) {
   if (c->scan.bad_k)
     return fail(PFDTD_E_DOMAIN, "%u inside voxels have K > 6 in their position byte",
                 c->scan.bad_k);
-  const bool literal = (c->flags & PFDTD_MATSLOT_LITERAL) != 0;
+  if (c->scheme == 2 && c->degenerate)
+    return fail(PFDTD_E_DOMAIN,
+                "%u inside voxels have no air neighbour on some axis (no centred stencil)",
+                c->degenerate);
+  if (c->scheme == 2 && c->lgB)
+    return fail(PFDTD_E_STATE, "source batching runs the forward update types only");
+  const bool literal = c->scheme == 0 && (c->flags & PFDTD_MATSLOT_LITERAL) != 0;
   for (int m = 0; m < 256; ++m) {
     if (!((c->scan.mat_present[m >> 5] >> (m & 31)) & 1u)) continue;
     if (material_slot((unsigned)m, c->octave, literal) >= (int64_t)c->n_mat * kCoefNum)
@@ -412,19 +458,19 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
   case PFDTD_F32:
     return launch_update_f32(g, UpdateArgs<float, CoefF, float>{c->d_pos, c->d_mat,
                              (const float *)c->d_P, (float *)c->d_Pp, c->d_coef_f, c->air_f,
-                             c->air_uniform, c->lam2_f}, c->stream);
+                             c->air_uniform, c->lam2_f, c->scheme}, c->stream);
   case PFDTD_F64:
     return launch_update_f64(g, UpdateArgs<double, CoefD, double>{c->d_pos, c->d_mat,
                              (const double *)c->d_P, (double *)c->d_Pp, c->d_coef_d, c->air_d,
-                             c->air_uniform, c->lam2_d}, c->stream);
+                             c->air_uniform, c->lam2_d, c->scheme}, c->stream);
   case PFDTD_F16:
     return launch_update_f16(g, UpdateArgs<h16, CoefF, float>{c->d_pos, c->d_mat,
                              (const h16 *)c->d_P, (h16 *)c->d_Pp, c->d_coef_f, c->air_f,
-                             c->air_uniform, c->lam2_f}, c->stream);
+                             c->air_uniform, c->lam2_f, c->scheme}, c->stream);
   case PFDTD_BF16:
     return launch_update_bf16(g, UpdateArgs<b16, CoefF, float>{c->d_pos, c->d_mat,
                               (const b16 *)c->d_P, (b16 *)c->d_Pp, c->d_coef_f, c->air_f,
-                              c->air_uniform, c->lam2_f}, c->stream);
+                              c->air_uniform, c->lam2_f, c->scheme}, c->stream);
   default:
     return hipErrorInvalidValue;
   }
@@ -433,7 +479,8 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
 // The two-level update needs the spare pair and no source on the never-updated slices
 // z = 0 and z = Z-1 (their values would have to be carried between the four buffers).
 bool fused_eligible(const pfdtd_ctx *c) {
-  if (c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2) return false;
+  if (c->scheme != 0 || c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
+    return false;
   for (int s = 0; s < c->n_src; ++s) {
     const int32_t *q = &c->src_xyz[3 * s];
     if (lin_idx(c, q) >= 0 && (q[2] == 0 || q[2] == c->Z - 1)) return false;
@@ -581,6 +628,7 @@ int pfdtd_set_domain(pfdtd_ctx *c, const uint8_t *pos, const uint8_t *mat, int64
   if (!pos || !mat) return fail(PFDTD_E_ARG, "null position or material array");
   rc = alloc_domain(c, X, Y, Z);
   if (rc) return rc;
+  c->degenerate = 0;
   HIP_TRY(hipMemcpy(c->d_pos, pos, (size_t)c->N, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_mat, mat, (size_t)c->N, hipMemcpyHostToDevice));
   return finish_domain(c);
@@ -599,10 +647,22 @@ int pfdtd_set_domain_labels(pfdtd_ctx *c, const uint8_t *labels, int64_t X, int6
     return fail(PFDTD_E_NOMEM, "cannot allocate the label grid");
   }
   hipError_t e = hipMemcpy(d_lab, labels, (size_t)c->N, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
+  unsigned *d_bad = nullptr;
+  c->degenerate = 0;
+  if (e == hipSuccess && c->scheme == 2) {
+    e = hipMalloc((void **)&d_bad, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(unsigned), c->stream);
+    if (e == hipSuccess)
+      e = launch_classify_labels_centred(d_lab, c->d_pos, c->d_mat, (int)X, (int)Y, (int)Z, d_bad,
+                                         c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&c->degenerate, d_bad, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+  } else if (e == hipSuccess) {
     e = launch_classify_labels(d_lab, c->d_pos, c->d_mat, (int)X, (int)Y, (int)Z, c->stream);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(d_lab);
+  if (d_bad) (void)hipFree(d_bad);
   if (e != hipSuccess) return fail(PFDTD_E_HIP, "label classification: %s", hipGetErrorString(e));
   return finish_domain(c);
 }
@@ -616,7 +676,11 @@ int pfdtd_set_domain_codes(pfdtd_ctx *c, const uint8_t *codes, const uint8_t *ma
   if (rc) return rc;
   HIP_TRY(hipMemcpy(c->d_pos, codes, (size_t)c->N, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_mat, mat, (size_t)c->N, hipMemcpyHostToDevice));
-  HIP_TRY(launch_codes_to_bilbao(c->d_pos, c->d_mat, c->N, c->stream));
+  c->degenerate = 0;
+  if (c->scheme == 2)
+    HIP_TRY(launch_codes_to_centred(c->d_pos, c->d_mat, c->N, c->stream));
+  else
+    HIP_TRY(launch_codes_to_bilbao(c->d_pos, c->d_mat, c->N, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return finish_domain(c);
 }
@@ -647,6 +711,26 @@ int pfdtd_get_counts(pfdtd_ctx *c, uint64_t *air, uint64_t *boundary) {
   return PFDTD_OK;
 }
 
+// Builds and uploads the coefficient tables of the current scheme and the air entry (K = 6 for
+// the forward scheme, D = 0 for the centred one, where it holds for every material byte).
+static int upload_coefficients(pfdtd_ctx *c) {
+  std::vector<CoefF> cf;
+  std::vector<CoefD> cd;
+  build_coefficients(c, cf, cd);
+  const size_t air_row = c->scheme == 2 ? 0 : 6 * 256;
+  c->air_f = cf[air_row];
+  c->air_d = cd[air_row];
+  c->air_uniform = 1;
+  for (int m = 0; m < 256; ++m) {
+    if (std::memcmp(&cf[air_row + m], &c->air_f, sizeof(CoefF)) != 0 ||
+        std::memcmp(&cd[air_row + m], &c->air_d, sizeof(CoefD)) != 0)
+      c->air_uniform = 0;
+  }
+  HIP_TRY(hipMemcpy(c->d_coef_f, cf.data(), sizeof(CoefF) * cf.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_coef_d, cd.data(), sizeof(CoefD) * cd.size(), hipMemcpyHostToDevice));
+  return PFDTD_OK;
+}
+
 int pfdtd_set_materials(pfdtd_ctx *c, const float *table, int n_mat, int n_coef,
                         unsigned octave, double lambda, int flags) {
   int rc = check_ctx(c);
@@ -668,20 +752,23 @@ int pfdtd_set_materials(pfdtd_ctx *c, const float *table, int n_mat, int n_coef,
   c->flags = flags;
   c->lam2_f = (float)(lambda * lambda);
   c->lam2_d = lambda * lambda;
-  std::vector<CoefF> cf;
-  std::vector<CoefD> cd;
-  build_coefficients(c, cf, cd);
-  c->air_f = cf[6 * 256];
-  c->air_d = cd[6 * 256];
-  c->air_uniform = 1;
-  for (int m = 0; m < 256; ++m) {
-    if (std::memcmp(&cf[6 * 256 + m], &c->air_f, sizeof(CoefF)) != 0 ||
-        std::memcmp(&cd[6 * 256 + m], &c->air_d, sizeof(CoefD)) != 0)
-      c->air_uniform = 0;
-  }
-  HIP_TRY(hipMemcpy(c->d_coef_f, cf.data(), sizeof(CoefF) * cf.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(c->d_coef_d, cd.data(), sizeof(CoefD) * cd.size(), hipMemcpyHostToDevice));
   c->has_materials = true;
+  return upload_coefficients(c);
+}
+
+int pfdtd_set_update_type(pfdtd_ctx *c, int type) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (type < 0 || type > 2) return fail(PFDTD_E_ARG, "update type %d not in {0,1,2}", type);
+  const int scheme = type == 2 ? 2 : 0;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->update_type = type;
+  if (scheme != c->scheme) {
+    // position bytes encode the boundary formulation: a classified domain no longer applies
+    c->scheme = scheme;
+    c->has_domain = false;
+    if (c->has_materials) return upload_coefficients(c);
+  }
   return PFDTD_OK;
 }
 

@@ -145,7 +145,43 @@ __device__ __forceinline__ C srl_node(unsigned pb, unsigned mb, C pc, C s, C pas
 // not a multiple of 256.
 constexpr int kTX = 256;
 
-template <typename S, int RY, int TW = kTX>
+// Centred-boundary node update, fdtd3dStdKowalczykMaterials (kernels3d.cu:609-665): the
+// position byte holds DIR_X/Y/Z (bits 0-2: the axis has one missing neighbour) and SIGN_X/Y/Z
+// (bits 4-6: which present neighbour stands in for it: +x, +y, and -z for SIGN_Z). The
+// coefficient entry of (number of DIR bits D, material) holds {1/(1+beta), beta-1, 2-6 lam2}
+// with beta = table * lambda * D, evaluated on the host in the reference's order.
+template <typename C>
+__device__ __forceinline__ C centred_node(unsigned pb, unsigned mb, C pc, C zp, C zm, C yp, C ym,
+                                          C xp, C xm, C past,
+                                          const typename CoefOf<C>::T *__restrict__ coef,
+                                          const typename CoefOf<C>::T &air, C lam2) {
+#pragma clang fp contract(off)
+  const unsigned dxb = pb & 1u, dyb = (pb >> 1) & 1u, dzb = (pb >> 2) & 1u;
+  typename CoefOf<C>::T e = air;
+  if (pb != 0x80u) e = coef[(dxb + dyb + dzb) * 256u + mb];
+  const C sx = (pb & 0x10u) ? xp : xm;  // p_x[sign_x]
+  const C sy = (pb & 0x20u) ? yp : ym;  // p_y[sign_y]
+  const C sz = (pb & 0x40u) ? zm : zp;  // p_z[sign_z]: p_z[1] is z-1
+  C sb = sx * (C)dxb;
+  sb = sb + sy * (C)dyb;
+  sb = sb + sz * (C)dzb;
+  // (p_x[0]+p_x[1]+p_y[0]+p_y[1]+p_z[0]+p_z[1]+S_boundary)*lambda^2, left to right
+  C s = xm + xp;
+  s = s + ym;
+  s = s + yp;
+  s = s + zp;
+  s = s + zm;
+  s = s + sb;
+  s = s * lam2;
+  C t = s - e.tk * pc;
+  t = t + e.omb * past;
+  t = t * (C)(pb >> 7);
+  return t * e.inv;
+}
+
+// SCHEME 0: forward (Bilbao) boundaries, update types SRL_FORWARD and SHARED; SCHEME 2: the
+// centred (Kowalczyk) boundaries of update type SRL.
+template <typename S, int RY, int TW = kTX, int SCHEME = 0>
 __device__ __forceinline__ void srl_tile(
     const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
     S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
@@ -203,16 +239,22 @@ __device__ __forceinline__ void srl_tile(
     for (int r = 0; r < RY; ++r) {
       const C yp = (r + 1 < RY) ? pc[r + 1] : yhi;
       const C ym = (r > 0) ? pc[r - 1] : ylo;
-      // Sum order z+1, z-1, y+1, y-1, x+1, x-1 (kernels3d.cu:516-525).
-      C s = pn[r] + pm[r];
-      s = s + yp;
-      s = s + ym;
-      s = s + xp[r];
-      s = s + xm[r];
-      typename CoefOf<C>::T e = air;
-      if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * 256u + mb[r]];
-      const C a = (C)(pb[r] >> 7) * e.inv;
-      const C ret = a * ((e.tk * pc[r] + lam2 * s) - e.omb * past[r]);
+      C ret;
+      if (SCHEME == 2) {
+        ret = centred_node<C>(pb[r], mb[r], pc[r], pn[r], pm[r], yp, ym, xp[r], xm[r], past[r],
+                              coef, air, lam2);
+      } else {
+        // Sum order z+1, z-1, y+1, y-1, x+1, x-1 (kernels3d.cu:516-525).
+        C s = pn[r] + pm[r];
+        s = s + yp;
+        s = s + ym;
+        s = s + xp[r];
+        s = s + xm[r];
+        typename CoefOf<C>::T e = air;
+        if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * 256u + mb[r]];
+        const C a = (C)(pb[r] >> 7) * e.inv;
+        ret = a * ((e.tk * pc[r] + lam2 * s) - e.omb * past[r]);
+      }
       if (live[r]) Fmt<S>::st(Pp, base + row[r], upd[r] ? ret : (C)0);
     }
 #pragma unroll

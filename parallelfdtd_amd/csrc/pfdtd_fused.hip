@@ -93,10 +93,23 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   __shared__ int s_rec[kFWaves][kFListCap];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int xb = blockIdx.x * kFO;  // first owned column
+  // XCD-aware tile order: consecutive block ids are dealt round-robin to the 8 XCDs (each with
+  // its own L2); remapping them so that every XCD walks a contiguous run of x-fastest tiles keeps
+  // the halo columns and rows that neighbouring tiles share in one L2 while both are resident.
+  int bxi, byi, bzi;
+  {
+    const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+    const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const unsigned full = total & ~7u;
+    const unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+    bxi = (int)(tile % nx);
+    byi = (int)((tile / nx) % ny);
+    bzi = (int)(tile / (nx * ny));
+  }
+  const int xb = bxi * kFO;  // first owned column
   const int x = xb - 2 + lane;      // this lane's column
-  const int yt = (blockIdx.y * kFWaves + wave) * kFTY;  // first owned row
-  const int z0 = 1 + (int)blockIdx.z * zc;
+  const int yt = (byi * kFWaves + wave) * kFTY;  // first owned row
+  const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);  // owned slices [z0, z1)
 
   // Sources that land on this wave's P^{n+1} region (owned tile + 1-voxel ring, slices

@@ -64,6 +64,7 @@ struct UpdateArgs {
   CT air;
   int air_uniform;
   C lam2;
+  int scheme;  // 0 forward (Bilbao) boundaries, 2 centred (Kowalczyk) boundaries
 };
 
 // Launchers (all asynchronous on `stream`; return the launch status).
@@ -156,7 +157,15 @@ hipError_t launch_classify_labels(const uint8_t *labels, uint8_t *pos, uint8_t *
                                   int Y, int Z, hipStream_t stream);
 hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream);
 hipError_t launch_scan_domain(const uint8_t *pos, const uint8_t *mat, int X, int Y, int Z,
-                              DomainScan *d_out, hipStream_t stream);
+                              int scheme, DomainScan *d_out, hipStream_t stream);
+// Centred-scheme (update type SRL) classification of labels / voxelizer codes.
+hipError_t launch_classify_labels_centred(const uint8_t *labels, uint8_t *pos, uint8_t *mat,
+                                          int X, int Y, int Z, unsigned int *degenerate,
+                                          hipStream_t stream);
+hipError_t launch_codes_to_centred(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream);
+// Centred-scheme coefficient tables: entry (D, m), D = number of DIR bits (0..3).
+void build_centred_tables(const float *table20, int n_mat, unsigned octave, double lambda,
+                          CoefF *cf, CoefD *cd);
 // Gathers a plane of the field (unscaled FP32) and of the position bytes.
 hipError_t launch_capture_slice(int precision, const void *field, const uint8_t *pos, int X,
                                 int Y, int Z, int64_t slice, int orientation, int scale_exp,

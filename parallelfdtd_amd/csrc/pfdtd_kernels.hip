@@ -17,14 +17,14 @@ namespace pfdtd {
 // ---------------------------------------------------------------------------------------
 // SRL_FORWARD update: one tile per block (srl_tile, pfdtd_device.h)
 // ---------------------------------------------------------------------------------------
-template <typename S, int RY>
+template <typename S, int RY, int SCHEME = 0>
 __global__ __launch_bounds__(kTX) void srl_update_kernel(
     const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
     S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
     typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
     int X, int Y, int Z, int zc) {
-  srl_tile<S, RY>(pos, mat, P, Pp, coef, air, air_uniform, lam2, X, Y, Z, zc, (int)blockIdx.x,
-                  (int)blockIdx.y, (int)blockIdx.z);
+  srl_tile<S, RY, kTX, SCHEME>(pos, mat, P, Pp, coef, air, air_uniform, lam2, X, Y, Z, zc,
+                               (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -142,6 +142,8 @@ UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc) {
 template <typename S, typename CT, typename C>
 static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> &a,
                                 hipStream_t stream) {
+  if ((a.scheme != 0 && a.scheme != 2) || (a.scheme == 2 && g.lgB > 0))
+    return hipErrorInvalidValue;
   if (g.lgB > 0) {
     switch (g.ry) {
 #define PFDTD_RYB_CASE(R)                                                                    \
@@ -155,6 +157,24 @@ static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> 
       PFDTD_RYB_CASE(4)
       PFDTD_RYB_CASE(8)
 #undef PFDTD_RYB_CASE
+    default:
+      return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  if (a.scheme == 2) {
+    switch (g.ry) {
+#define PFDTD_RYC_CASE(R)                                                                    \
+  case R:                                                                                    \
+    hipLaunchKernelGGL((srl_update_kernel<S, R, 2>), g.grid, g.block, 0, stream, a.pos,      \
+                       a.mat, a.P, a.Pp, a.coef, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z, \
+                       g.zc);                                                                \
+    break;
+      PFDTD_RYC_CASE(1)
+      PFDTD_RYC_CASE(2)
+      PFDTD_RYC_CASE(4)
+      PFDTD_RYC_CASE(8)
+#undef PFDTD_RYC_CASE
     default:
       return hipErrorInvalidValue;
     }
@@ -273,6 +293,114 @@ hipError_t launch_classify_labels(const uint8_t *labels, uint8_t *pos, uint8_t *
   return hipGetLastError();
 }
 
+// Centred-scheme (update type SRL) position byte from the set of air face neighbours
+// (bit 0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z): an axis with exactly one air neighbour gets its
+// DIR bit (0x01 / 0x02 / 0x04) and a SIGN bit (0x10 / 0x20 / 0x40) naming the present neighbour
+// the update mirrors (+x, +y, and -z for SIGN_Z), cudaMesh.h:35-44, kernels3d.cu:644-658.
+// Returns 0 when some axis has no air neighbour (no centred stencil exists).
+__device__ __forceinline__ unsigned centred_byte(unsigned air) {
+  unsigned pb = 0x80u;
+  const unsigned ax = air & 3u, ay = (air >> 2) & 3u, az = (air >> 4) & 3u;
+  if (!ax || !ay || !az) return 0u;
+  if (ax == 1u) pb |= 0x01u;
+  if (ax == 2u) pb |= 0x11u;
+  if (ay == 1u) pb |= 0x02u;
+  if (ay == 2u) pb |= 0x22u;
+  if (az == 1u) pb |= 0x44u;
+  if (az == 2u) pb |= 0x04u;
+  return pb;
+}
+
+// Label grid -> centred-scheme bytes (same material rule as classify_labels_kernel); voxels
+// without a centred stencil are counted in `degenerate` (one atomic per wavefront).
+__global__ __launch_bounds__(256) void classify_labels_centred_kernel(
+    const uint8_t *__restrict__ label, uint8_t *__restrict__ pos, uint8_t *__restrict__ mat,
+    int X, int Y, int Z, unsigned int *degenerate) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int y = blockIdx.y;
+  const int z = blockIdx.z;
+  bool bad = false;
+  if (x < X) {
+    const int64_t XY = (int64_t)X * Y;
+    const int64_t i = (int64_t)z * XY + (int64_t)y * X + x;
+    if (label[i] != 0) {
+      pos[i] = 0;
+      mat[i] = 0;
+    } else {
+      const bool inb[6] = {x > 0, x < X - 1, y > 0, y < Y - 1, z > 0, z < Z - 1};
+      const int64_t off[6] = {-1, 1, -(int64_t)X, (int64_t)X, -XY, XY};
+      unsigned air = 0, m = 0;
+      bool found = false;
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        const unsigned lab = inb[d] ? label[i + off[d]] : 1u;
+        if (lab == 0)
+          air |= 1u << d;
+        else if (!found) {
+          found = true;
+          m = inb[d] ? lab - 1u : 0u;
+        }
+      }
+      unsigned pb = centred_byte(air);
+      if (!pb) {
+        bad = true;
+        // keep the DIR/SIGN bits of the axes that have a stencil
+        pb = 0x80u;
+        const unsigned ax = air & 3u, ay = (air >> 2) & 3u, az = (air >> 4) & 3u;
+        if (ax == 1u) pb |= 0x01u;
+        if (ax == 2u) pb |= 0x11u;
+        if (ay == 1u) pb |= 0x02u;
+        if (ay == 2u) pb |= 0x22u;
+        if (az == 1u) pb |= 0x44u;
+        if (az == 2u) pb |= 0x04u;
+      }
+      pos[i] = (uint8_t)pb;
+      mat[i] = (uint8_t)(air == 0x3Fu ? 0u : m);
+    }
+  }
+  const uint64_t b = __ballot(bad);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(degenerate, (unsigned)__popcll(b));
+}
+
+hipError_t launch_classify_labels_centred(const uint8_t *labels, uint8_t *pos, uint8_t *mat,
+                                          int X, int Y, int Z, unsigned int *degenerate,
+                                          hipStream_t stream) {
+  const dim3 grid((X + 255) / 256, Y, Z), block(256);
+  hipLaunchKernelGGL(classify_labels_centred_kernel, grid, block, 0, stream, labels, pos, mat, X,
+                     Y, Z, degenerate);
+  return hipGetLastError();
+}
+
+// toKowalczyk (src/kernels/cudaMesh.cu:363-480) applied in place to voxelizer codes: each code
+// names its air directions; the byte follows from them as in centred_byte.
+__global__ __launch_bounds__(256) void codes_to_centred_kernel(uint8_t *pos, uint8_t *mat,
+                                                               int64_t n) {
+  // air-direction sets of codes 1..27 (bit 0 -x "Left", 1 +x "Right", 2 -y "In", 3 +y "Out",
+  // 4 -z "Down", 5 +z "Up")
+  constexpr uint8_t kAir[28] = {
+      0x00, 0x15, 0x16, 0x19, 0x1A, 0x25, 0x26, 0x29, 0x2A, 0x17, 0x1B, 0x1D, 0x1E, 0x27,
+      0x2B, 0x2D, 0x2E, 0x35, 0x36, 0x39, 0x3A, 0x1F, 0x3B, 0x37, 0x3E, 0x3D, 0x2F, 0x3F};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const unsigned k = pos[i];
+    if (k == 0) {
+      pos[i] = 0;
+      mat[i] = 0;
+    } else if (k <= 27) {
+      pos[i] = (uint8_t)centred_byte(kAir[k]);
+    }
+  }
+}
+
+hipError_t launch_codes_to_centred(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream) {
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(codes_to_centred_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, pos,
+                     mat, n);
+  return hipGetLastError();
+}
+
 // toBilbao (src/kernels/cudaMesh.cu:328-361) applied in place to voxelizer codes.
 __global__ __launch_bounds__(256) void codes_to_bilbao_kernel(uint8_t *pos, uint8_t *mat,
                                                               int64_t n) {
@@ -303,7 +431,8 @@ hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStre
   return hipGetLastError();
 }
 
-// Counts air/boundary voxels (calcBoundaries, cudaMesh.cu:500-514) and checks the
+// Counts air/boundary voxels (calcBoundaries, cudaMesh.cu:500-514; air value 0x86 for the
+// forward scheme, 0x80 for the centred one, cudaMesh.cu:176-186) and checks the
 // preconditions the update relies on. Blocks stride over (y, z) rows, threads over x; each
 // thread keeps private counters and a 64-bit set of the material bytes below 64 it saw, the
 // wavefront reduces them with cross-lane shuffles, and only one atomic per wavefront and
@@ -321,7 +450,8 @@ __device__ __forceinline__ unsigned long long wave_or(unsigned long long v) {
 
 __global__ __launch_bounds__(256) void scan_domain_kernel(const uint8_t *__restrict__ pos,
                                                           const uint8_t *__restrict__ mat, int X,
-                                                          int Y, int Z, DomainScan *out) {
+                                                          int Y, int Z, unsigned air_value,
+                                                          int check_k, DomainScan *out) {
   __shared__ unsigned long long s_cnt[4];
   __shared__ unsigned int s_mat[8];
   if (threadIdx.x < 8) s_mat[threadIdx.x] = 0;
@@ -338,10 +468,10 @@ __global__ __launch_bounds__(256) void scan_domain_kernel(const uint8_t *__restr
     for (int x = threadIdx.x; x < X; x += 256) {
       const unsigned p = prow[x];
       const unsigned m = mrow[x];
-      air += (p == 0x86u);
-      bnd += (p != 0u && p != 0x86u);
+      air += (p == air_value);
+      bnd += (p != 0u && p != air_value);
       if (p >> 7) {
-        badk += ((p & 0x7Fu) > 6u);
+        badk += check_k && ((p & 0x7Fu) > 6u);
         shell += (edge_row || x == 0 || x == X - 1);
       }
       if (m < 64u)
@@ -374,11 +504,12 @@ __global__ __launch_bounds__(256) void scan_domain_kernel(const uint8_t *__restr
 }
 
 hipError_t launch_scan_domain(const uint8_t *pos, const uint8_t *mat, int X, int Y, int Z,
-                              DomainScan *d_out, hipStream_t stream) {
+                              int scheme, DomainScan *d_out, hipStream_t stream) {
   const int64_t rows = (int64_t)Y * Z;
   const int blocks = (int)std::min<int64_t>(rows, 8192);
+  const unsigned air_value = scheme == 2 ? 0x80u : 0x86u;
   hipLaunchKernelGGL(scan_domain_kernel, dim3(blocks), dim3(256), 0, stream, pos, mat, X, Y, Z,
-                     d_out);
+                     air_value, scheme == 2 ? 0 : 1, d_out);
   return hipGetLastError();
 }
 

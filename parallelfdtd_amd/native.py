@@ -18,6 +18,7 @@ E_ARG, E_HIP, E_NOMEM, E_STATE, E_DOMAIN = -1, -2, -3, -4, -5
 
 F32, F64, F16, BF16 = 0, 1, 2, 3
 SRC_HARD, SRC_SOFT, SRC_TRANSPARENT = 0, 1, 2
+SRL_FORWARD, SHARED, SRL = 0, 1, 2  # enum UpdateType
 MATSLOT_INTENDED, MATSLOT_LITERAL = 0, 1
 SOFT_ADD, SOFT_OVERWRITE = 0, 2
 
@@ -71,6 +72,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_kernel_timing": [_p, C.c_int],
     "pfdtd_get_kernel_timing": [_p, _pdbl, _pi64],
     "pfdtd_set_poll_interval": [_p, C.c_int],
+    "pfdtd_set_update_type": [_p, C.c_int],
     "pfdtd_set_batch": [_p, C.c_int],
     "pfdtd_set_source_slots": [_p, C.c_int, _p],
     "pfdtd_set_receiver_slots": [_p, C.c_int, _p],

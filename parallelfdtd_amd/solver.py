@@ -65,7 +65,8 @@ class Solver:
         self.dims = (X, Y, Z)
 
     def set_domain_codes(self, codes: np.ndarray, mat: np.ndarray) -> None:
-        """(Z, Y, X) voxelizer node codes 0..27 and material bytes (toBilbao on the GPU)."""
+        """(Z, Y, X) voxelizer node codes 0..27 and material bytes (toBilbao, or toKowalczyk for
+        update type SRL, on the GPU)."""
         codes = np.ascontiguousarray(codes, np.uint8)
         mat = np.ascontiguousarray(mat, np.uint8)
         Z, Y, X = codes.shape
@@ -122,6 +123,10 @@ class Solver:
 
     def set_update_tiling(self, rows: int = 1, zchunk: int = 8) -> None:
         self._c("pfdtd_set_update_tiling", rows, zchunk)
+
+    def set_update_type(self, update_type: int) -> None:
+        """0 SRL_FORWARD, 1 SHARED (same update), 2 SRL (centred boundaries); before the domain."""
+        self._c("pfdtd_set_update_type", update_type)
 
     def set_batch(self, slots: int) -> None:
         self._c("pfdtd_set_batch", slots)
