@@ -39,7 +39,8 @@ from parallelfdtd_amd.solver import Solver  # noqa: E402
 METRIC = "Gvoxel-updates/s + % HBM roofline (FP32 vs FP16-store); p50/p99 IR latency"
 PEAK_GBS = 8000.0
 BYTES_PER_NODE = {N.F32: 14, N.F16: 8, N.BF16: 8, N.F64: 26}
-TRAFFIC_SUMMARY = ROOT / "profiles" / "srl_update_c3_f32_hbm_traffic.md"
+# HBM bytes per launch of the dominant FP32 kernel (two time levels per launch), from counters
+TRAFFIC_SUMMARY = ROOT / "profiles" / "srl_update2_c3_f32_hbm_traffic.md"
 
 
 def log(msg: str) -> None:
@@ -205,7 +206,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel_ms": round(kms, 4), "bytes_per_node": BYTES_PER_NODE[N.F32]},
+                     "kernel": "srl_update2_kernel<float> (two time levels per launch)",
+                     "launch_ms": round(2 * kms, 4), "ms_per_step": round(kms, 4),
+                     "bytes_per_node_update": BYTES_PER_NODE[N.F32]},
         "fp16_storage": fp16,
     }
     if args.serve_jobs > 0:

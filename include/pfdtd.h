@@ -192,9 +192,9 @@ int pfdtd_set_kernel_timing(pfdtd_ctx *ctx, int enabled);
  * PROGRESS_MOD in src/kernels/kernels3d.h:36). The reference polls the interrupt every step
  * (kernels3d.cu:86); polling here costs one stream synchronisation per interval. */
 int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
-/* Time levels advanced per sweep of the field (1 or 2; 0 = default: 1 for FP32, 2 for FP64 and
- * the 2-byte storage formats, the faster path for each on MI355X) and slices per z-chunk of the
- * two-level update (default 32). With 2, pfdtd_run advances pairs of steps in one kernel that
+/* Time levels advanced per sweep of the field (1 or 2; 0 = default = 2, the faster path on
+ * MI355X in every precision) and slices per z-chunk of the two-level update (default 32). The
+ * centred update type and batched contexts always advance one level per sweep. With 2, pfdtd_run advances pairs of steps in one kernel that
  * reads P^n, P^{n-1} and the byte planes once and writes P^{n+1}, P^{n+2} (four field buffers;
  * sources of step n+1 injected and receivers of step n gathered inside the kernel), falling back
  * to single steps for an odd remainder and whenever a source sits on slice z = 0 or z = Z-1.
@@ -218,7 +218,8 @@ int pfdtd_get_kernel_timing(pfdtd_ctx *ctx, double *update_ms_total, int64_t *up
  * until a job joins or leaves. Jobs are admitted while their buffers fit in the arena's free
  * space and a slot is free (at most 64 resident), so a job submitted while others run starts
  * at the next bundle and overlaps them on the device. Every job's arithmetic is that of a
- * single-context run of the same inputs (bit-identical responses).
+ * single-context run of the same inputs (bit-identical responses). A pool is driven by one
+ * host thread at a time (calls on one pool must not overlap); pools are independent.
  * No reference counterpart: the reference runs one simulation per App (App::runSimulation,
  * src/App.cpp:308-348); a job's inputs and semantics are those of that call with the voxel
  * entry point, sources as App::addSource (src/App.h) with DATA signals, and receivers as

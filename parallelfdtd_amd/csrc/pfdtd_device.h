@@ -284,25 +284,30 @@ __device__ __forceinline__ void inject_gather_block(S *field, const InjectArgs &
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && a.inject_step >= 0) {
-    const C scale = (C)ldexp(1.0, a.scale_exp);
-    for (int s = 0; s < a.n_src; ++s) {
-      const int64_t idx = a.src_idx[s];
-      if (idx < 0) continue;
-      C smp = (C)0;
-      if (a.inject_step < a.n_samples) {
-        const int64_t off = (int64_t)s * a.n_samples + a.inject_step;
-        if (sizeof(C) == sizeof(double))
-          smp = (C)a.sig_d[off];
-        else
-          smp = (C)a.sig_f[off];
-      }
-      smp = smp * scale;
-      if (a.src_kind[s] == 0 || a.soft_overwrite)
-        Fmt<S>::st(field, idx, smp);
+  if (a.inject_step < 0) return;
+  const C scale = (C)ldexp(1.0, a.scale_exp);
+  auto apply = [&](int s) {
+    const int64_t idx = a.src_idx[s];
+    if (idx < 0) return;
+    C smp = (C)0;
+    if (a.inject_step < a.n_samples) {
+      const int64_t off = (int64_t)s * a.n_samples + a.inject_step;
+      if (sizeof(C) == sizeof(double))
+        smp = (C)a.sig_d[off];
       else
-        Fmt<S>::st(field, idx, Fmt<S>::ld(field, idx) + smp);
+        smp = (C)a.sig_f[off];
     }
+    smp = smp * scale;
+    if (a.src_kind[s] == 0 || a.soft_overwrite)
+      Fmt<S>::st(field, idx, smp);
+    else
+      Fmt<S>::st(field, idx, Fmt<S>::ld(field, idx) + smp);
+  };
+  if (a.distinct) {
+    // distinct elements: the order of application cannot change any value
+    for (int s = threadIdx.x; s < a.n_src; s += blockDim.x) apply(s);
+  } else if (threadIdx.x == 0) {
+    for (int s = 0; s < a.n_src; ++s) apply(s);  // index order: the last hard source wins
   }
 }
 

@@ -140,6 +140,7 @@ struct InjectArgs {
   int inject_step;
   int soft_overwrite;
   int scale_exp;
+  int distinct;  // 1: no two sources share an element, so they may be applied in parallel
 };
 hipError_t launch_inject_gather(int precision, void *field, const InjectArgs &a,
                                 hipStream_t stream);

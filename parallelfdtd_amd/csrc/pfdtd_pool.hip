@@ -53,6 +53,7 @@ struct PoolJobDev {
   const float *sig;  // [n_src][n_samples]
   int n_rec, n_src, n_samples, n_steps;
   int soft_overwrite, scale_exp;
+  int distinct;  // sources on distinct voxels: injection may run in parallel
 };
 
 struct PoolState {
@@ -118,6 +119,7 @@ __global__ __launch_bounds__(256) void pool_inject_kernel(const PoolJobDev *__re
   a.inject_step = (J.n_src > 0 && st.step + 1 < J.n_steps) ? st.step + 1 : -1;
   a.soft_overwrite = J.soft_overwrite;
   a.scale_exp = J.scale_exp;
+  a.distinct = J.distinct;
   // the update just wrote the new field into the past buffer
   inject_gather_block<S>((S *)(st.parity ? J.A : J.B), a);
   __syncthreads();
@@ -646,6 +648,13 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   d.n_steps = j->n_steps;
   d.soft_overwrite = (j->flags & PFDTD_SOFT_OVERWRITE) ? 1 : 0;
   d.scale_exp = j->scale_exp;
+  {
+    std::vector<int64_t> v;
+    for (int64_t i : sidx)
+      if (i >= 0) v.push_back(i);
+    std::sort(v.begin(), v.end());
+    d.distinct = std::adjacent_find(v.begin(), v.end()) == v.end() ? 1 : 0;
+  }
   const PoolState st0{0, 0};
 
   // set-up on the upload stream, concurrent with the resident jobs' steps
@@ -686,6 +695,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
     a.inject_step = 0;
     a.soft_overwrite = d.soft_overwrite;
     a.scale_exp = j->scale_exp;
+    a.distinct = d.distinct;
     e = launch_inject_gather(j->precision, d.A, a, u);
   }
   if (e == hipSuccess) e = hipMemcpyAsync(p->d_jobs + slot, &d, sizeof(d), hipMemcpyHostToDevice, u);

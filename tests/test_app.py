@@ -166,3 +166,24 @@ def test_run_capture_writes_slices_and_meshes(gpu, tmp_path):
                          src[:0], 25, want_fields=True)
     assert np.array_equal(app.getMeshCaptureAt(0), P)
     app.close()
+
+
+@pytest.mark.gpu
+def test_update_type_srl_reclassifies_labels(gpu):
+    """setUpdateType(2) after the label domain is set: the App reclassifies it for the centred
+    boundaries at initializeMesh, as the reference classifies when the mesh is set up."""
+    n_steps = 200
+    app, lab, betas = _room_app(fs=40000, n_steps=n_steps)
+    app.setUpdateType(2)
+    app.addSource(0.15, 0.12, 0.10, 1, 1, 0)
+    app.addReceiver(0.2, 0.15, 0.12)
+    app.runSimulation()
+    pos, mat, counts = oracle.classify_kow(lab)
+    assert counts[2] == 0
+    src = np.array([app.getSourceElementCoordinates(0)], np.int32)
+    rec = np.array([app.getReceiverElementCoordinates(0)], np.int32)
+    sig = np.array([[app.getSourceSample(0, n) for n in range(n_steps)]], np.float32)
+    ref = oracle.run(pos, mat, np.repeat(betas[:, None], 20, axis=1), src, np.array([1], np.int32),
+                     sig, rec, n_steps, literal=oracle.SCHEME_KOWALCZYK)
+    assert np.array_equal(app.getResponse(0), ref[0])
+    app.close()
