@@ -117,6 +117,16 @@ int pfdtd_set_sources(pfdtd_ctx *ctx, int n_src, const int32_t *xyz, const int32
  * SimulationParameters.cpp:213-224). Receivers outside the grid record 0 (kernels3d.cu:168). */
 int pfdtd_set_receivers(pfdtd_ctx *ctx, int n_rec, const int32_t *xyz);
 
+/* ---- transparent sources -------------------------------------------------------------- */
+/* out[s][n] = in[s][n] - sum_{i<n} h[n-i] in[s][i] for n_src signals of n_samples samples, on
+ * the device: getTransparentSourceSample(+Double), SimulationParameters.cpp:300-310, :350-360
+ * (the same left-to-right sum per sample, FP32 or FP64; h = the FP32 grid impulse response,
+ * readGridIr, 0 past n_ir). Either precision pair may be null. The reference evaluates this
+ * O(n^2) sum on the host inside its step loop. */
+int pfdtd_transparent_signals(pfdtd_ctx *ctx, int n_src, int n_samples, const float *grid_ir,
+                              int n_ir, const float *in_f32, float *out_f32,
+                              const double *in_f64, double *out_f64);
+
 /* ---- update type -------------------------------------------------------------------- */
 /* enum UpdateType (src/base/SimulationParameters.h:39): 0 SRL_FORWARD (forward boundaries,
  * fdtd3dStdMaterials, kernels3d.cu:485-529; the default), 1 SHARED (the same update: the

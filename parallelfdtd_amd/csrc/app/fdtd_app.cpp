@@ -425,9 +425,36 @@ void App::uploadSourcesAndReceivers(bool single_precision_only) {
     sxyz[3 * s + 1] = q.y;
     sxyz[3 * s + 2] = q.z;
     kind[s] = (int32_t)m_parameters.getSource(s).getSourceType();
+    // regular samples here; transparent sources are filtered on the device below (the same
+    // values getSourceSample(+Double) returns, without the O(steps^2) host loop)
     for (unsigned int n = 0; n < steps; ++n) {
-      sf[(size_t)s * steps + n] = m_parameters.getSourceSample(s, n);
-      if (!single_precision_only) sd[(size_t)s * steps + n] = m_parameters.getSourceSampleDouble(s, n);
+      sf[(size_t)s * steps + n] = 0.f + m_parameters.getRegularSourceSample(s, n);
+      if (!single_precision_only)
+        sd[(size_t)s * steps + n] = 0.0 + m_parameters.getRegularSourceSampleDouble(s, n);
+    }
+  }
+  std::vector<unsigned int> transparent;
+  for (unsigned int s = 0; s < ns; ++s)
+    if (kind[s] == (int32_t)SRC_TRANSPARENT) transparent.push_back(s);
+  if (!transparent.empty() && steps) {
+    const std::vector<float> &h = m_parameters.getGridIr();
+    const size_t m = transparent.size();
+    std::vector<float> inf(m * steps), outf(m * steps);
+    std::vector<double> ind(single_precision_only ? 0 : m * steps), outd(ind.size());
+    for (size_t j = 0; j < m; ++j) {
+      std::copy_n(&sf[(size_t)transparent[j] * steps], steps, &inf[j * steps]);
+      if (!single_precision_only)
+        std::copy_n(&sd[(size_t)transparent[j] * steps], steps, &ind[j * steps]);
+    }
+    check(pfdtd_transparent_signals(ctx_, (int)m, (int)steps, h.empty() ? nullptr : h.data(),
+                                    (int)h.size(), inf.data(), outf.data(),
+                                    single_precision_only ? nullptr : ind.data(),
+                                    single_precision_only ? nullptr : outd.data()),
+          "transparentSources");
+    for (size_t j = 0; j < m; ++j) {
+      std::copy_n(&outf[j * steps], steps, &sf[(size_t)transparent[j] * steps]);
+      if (!single_precision_only)
+        std::copy_n(&outd[j * steps], steps, &sd[(size_t)transparent[j] * steps]);
     }
   }
   for (unsigned int r = 0; r < nr; ++r) {

@@ -132,6 +132,14 @@ class SimulationParameters {
   unsigned int getNumReceivers() const { return (unsigned int)receivers_.size(); }
   Source getSource(unsigned int i) const { return sources_.at(i); }
   Receiver getReceiver(unsigned int i) const { return receivers_.at(i); }
+  // getRegularSourceSample(+Double), SimulationParameters.cpp:262-298, :312-348
+  float getRegularSourceSample(unsigned int src, unsigned int step) const {
+    return regularSample(src, step);
+  }
+  double getRegularSourceSampleDouble(unsigned int src, unsigned int step) const {
+    return regularSampleDouble(src, step);
+  }
+  const std::vector<float> &getGridIr() const { return grid_ir_; }
   std::vector<float> getParameterVector() const;        // getParameterPtr, :379-387
   std::vector<double> getParameterVectorDouble() const;  // getParameterPtrDouble, :389-396
 

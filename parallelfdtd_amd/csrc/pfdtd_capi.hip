@@ -769,6 +769,43 @@ int pfdtd_set_materials(pfdtd_ctx *c, const float *table, int n_mat, int n_coef,
   return upload_coefficients(c);
 }
 
+int pfdtd_transparent_signals(pfdtd_ctx *c, int n_src, int n_samples, const float *grid_ir,
+                              int n_ir, const float *in_f32, float *out_f32,
+                              const double *in_f64, double *out_f64) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n_src < 0 || n_samples < 0 || n_ir < 0) return fail(PFDTD_E_ARG, "negative count");
+  if (n_ir > 0 && !grid_ir) return fail(PFDTD_E_ARG, "null grid impulse response");
+  if ((in_f32 == nullptr) != (out_f32 == nullptr) || (in_f64 == nullptr) != (out_f64 == nullptr))
+    return fail(PFDTD_E_ARG, "each precision needs both an input and an output array");
+  const size_t n = (size_t)n_src * (size_t)n_samples;
+  if (!n) return PFDTD_OK;
+  float *d_h = nullptr;
+  void *d_in = nullptr, *d_out = nullptr;
+  hipError_t e = hipMalloc((void **)&d_h, sizeof(float) * (size_t)(n_ir > 0 ? n_ir : 1));
+  if (e == hipSuccess && n_ir > 0)
+    e = hipMemcpy(d_h, grid_ir, sizeof(float) * (size_t)n_ir, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&d_in, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * n);
+  for (int pass = 0; pass < 2 && e == hipSuccess; ++pass) {
+    const bool f64 = pass == 1;
+    const void *in = f64 ? (const void *)in_f64 : (const void *)in_f32;
+    void *out = f64 ? (void *)out_f64 : (void *)out_f32;
+    if (!in) continue;
+    const size_t bytes = (f64 ? sizeof(double) : sizeof(float)) * n;
+    e = hipMemcpy(d_in, in, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = launch_transparent(f64, d_in, d_h, n_ir, n_src, n_samples, d_out, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, bytes, hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(d_h);
+  if (d_in) (void)hipFree(d_in);
+  if (d_out) (void)hipFree(d_out);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "transparent signals: %s", hipGetErrorString(e));
+  return PFDTD_OK;
+}
+
 int pfdtd_set_update_type(pfdtd_ctx *c, int type) {
   int rc = check_ctx(c);
   if (rc) return rc;

@@ -164,6 +164,9 @@ hipError_t launch_classify_labels_centred(const uint8_t *labels, uint8_t *pos, u
                                           int X, int Y, int Z, unsigned int *degenerate,
                                           hipStream_t stream);
 hipError_t launch_codes_to_centred(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream);
+// Transparent-source filter of n_src signals of n_samples each (FP32 or FP64).
+hipError_t launch_transparent(bool f64, const void *in, const float *h, int n_ir, int n_src,
+                              int n_samples, void *out, hipStream_t stream);
 // Centred-scheme coefficient tables: entry (D, m), D = number of DIR bits (0..3).
 void build_centred_tables(const float *table20, int n_mat, unsigned octave, double lambda,
                           CoefF *cf, CoefD *cd);

@@ -54,7 +54,6 @@ __global__ __launch_bounds__(kTX) void srl_update_batch_kernel(
   const int64_t sX = X;
   const int64_t XY = (int64_t)X * (int64_t)Y;
   const int64_t eX = (int64_t)B;           // element stride of x
-  const int64_t eY = sX * B;               // of y
   const int64_t eZ = XY * B;               // of z
   const bool xin = (x > 0) && (x < X - 1);
 
@@ -93,7 +92,6 @@ __global__ __launch_bounds__(kTX) void srl_update_batch_kernel(
     }
     const C ylo = Fmt<S>::ld(P, (base + halo_lo) * B + b);
     const C yhi = Fmt<S>::ld(P, (base + halo_hi) * B + b);
-    (void)eY;
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const C yp = (r + 1 < RY) ? pc[r + 1] : yhi;
@@ -245,6 +243,43 @@ hipError_t launch_inject_gather(int precision, void *field, const InjectArgs &a,
   default:
     return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Transparent-source signals
+// ---------------------------------------------------------------------------------------
+// out[s][n] = in[s][n] - sum_{i<n} h[n-i] in[s][i] (getTransparentSourceSample(+Double),
+// SimulationParameters.cpp:300-310, :350-360): one thread per sample, the sum accumulated left
+// to right in T exactly as the reference's host loop; h is the FP32 grid impulse response
+// (0 past its end).
+template <typename T>
+__global__ __launch_bounds__(256) void transparent_kernel(const T *__restrict__ in,
+                                                          const float *__restrict__ h, int n_ir,
+                                                          int n_samples, T *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  const int s = blockIdx.y;
+  if (n >= n_samples) return;
+  const T *x = in + (int64_t)s * n_samples;
+  T acc = (T)0;
+  for (int i = 0; i < n; ++i) {
+    const int k = n - i;
+    const T hk = k < n_ir ? (T)h[k] : (T)0;
+    acc += hk * x[i];
+  }
+  out[(int64_t)s * n_samples + n] = (T)0 + (x[n] - acc);
+}
+
+hipError_t launch_transparent(bool f64, const void *in, const float *h, int n_ir, int n_src,
+                              int n_samples, void *out, hipStream_t stream) {
+  const dim3 grid((n_samples + 255) / 256, n_src), block(256);
+  if (f64)
+    hipLaunchKernelGGL(transparent_kernel<double>, grid, block, 0, stream, (const double *)in, h,
+                       n_ir, n_samples, (double *)out);
+  else
+    hipLaunchKernelGGL(transparent_kernel<float>, grid, block, 0, stream, (const float *)in, h,
+                       n_ir, n_samples, (float *)out);
   return hipGetLastError();
 }
 

@@ -73,6 +73,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_get_kernel_timing": [_p, _pdbl, _pi64],
     "pfdtd_set_poll_interval": [_p, C.c_int],
     "pfdtd_set_update_type": [_p, C.c_int],
+    "pfdtd_transparent_signals": [_p, C.c_int, C.c_int, _p, C.c_int, _p, _p, _p, _p],
     "pfdtd_set_batch": [_p, C.c_int],
     "pfdtd_set_source_slots": [_p, C.c_int, _p],
     "pfdtd_set_receiver_slots": [_p, C.c_int, _p],

@@ -124,6 +124,21 @@ class Solver:
     def set_update_tiling(self, rows: int = 1, zchunk: int = 8) -> None:
         self._c("pfdtd_set_update_tiling", rows, zchunk)
 
+    def transparent_signals(self, signals: np.ndarray, grid_ir: np.ndarray) -> np.ndarray:
+        """Transparent-source filtering on the device (FP32 for float32 input, else FP64)."""
+        x = np.ascontiguousarray(signals)
+        f64 = x.dtype != np.float32
+        x = np.ascontiguousarray(x, np.float64 if f64 else np.float32).reshape(len(x), -1)
+        h = np.ascontiguousarray(grid_ir, np.float32)
+        out = np.empty_like(x)
+        if f64:
+            self._c("pfdtd_transparent_signals", x.shape[0], x.shape[1], N.ptr(h), h.size, None,
+                    None, N.ptr(x), N.ptr(out))
+        else:
+            self._c("pfdtd_transparent_signals", x.shape[0], x.shape[1], N.ptr(h), h.size,
+                    N.ptr(x), N.ptr(out), None, None)
+        return out
+
     def set_update_type(self, update_type: int) -> None:
         """0 SRL_FORWARD, 1 SHARED (same update), 2 SRL (centred boundaries); before the domain."""
         self._c("pfdtd_set_update_type", update_type)

@@ -187,3 +187,20 @@ def test_update_type_srl_reclassifies_labels(gpu):
                      sig, rec, n_steps, literal=oracle.SCHEME_KOWALCZYK)
     assert np.array_equal(app.getResponse(0), ref[0])
     app.close()
+
+
+@pytest.mark.gpu
+def test_transparent_source_through_app(gpu):
+    """A transparent source: the App filters its signal on the device; the responses equal the
+    oracle run driven by the App's own host-side getSourceSample values."""
+    n_steps = 150
+    app, lab, betas = _room_app(fs=40000, n_steps=n_steps)
+    h = np.zeros(200, np.float32)
+    h[2::2] = (-1.0 / 3.0) * (0.95 ** np.arange(99))
+    app.setGridIr(h.tolist())
+    app.addSource(0.15, 0.12, 0.10, 2, 1, 0)    # transparent, Gaussian input
+    app.addReceiver(0.2, 0.15, 0.12)
+    app.runSimulation()
+    ref = _oracle_responses(app, lab, betas, n_steps, "f32")
+    assert np.array_equal(app.getResponse(0), ref[0])
+    app.close()
