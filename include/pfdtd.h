@@ -87,6 +87,26 @@ int pfdtd_set_domain_labels(pfdtd_ctx *ctx, const uint8_t *labels, int64_t X, in
  * (toBilbaoKernel :492-498); material bytes as given (zeroed for code 0). */
 int pfdtd_set_domain_codes(pfdtd_ctx *ctx, const uint8_t *codes, const uint8_t *mat, int64_t X,
                            int64_t Y, int64_t Z);
+/* Triangle-mesh solid voxelization on the GPU, then classification for the current update
+ * type: replaces voxelizeGeometry (src/kernels/voxelizationUtils.cu:58-131, the external
+ * hakarlss/Voxelizer) + padWithZeros + toBilbaoScheme/toKowalczykScheme as App::initializeMesh
+ * runs them (src/App.cpp:178-211, src/kernels/cudaMesh.cu:26-153). vertices [n_vertices][3]
+ * (metres), indices [n_triangles][3], tri_material [n_triangles] (unique material index per
+ * triangle, MaterialHandler::getMaterialIdxPtr; null = all 0; at most 254), dx = voxel edge.
+ * The mesh must be closed (README.md:100): a node is air when the ray along +x through its
+ * centre has crossed the surface an odd number of times. Frame and grid: vertices minus their
+ * per-axis minimum; node (i,j,k) at ((i-1)dx, (j-1)dx, (k-1)dx) (the +1 padding offset of
+ * Position::getElementIdx, src/base/SrcRec.cpp:26-34); floor(extent/dx)+3 nodes per axis,
+ * rounded up to a multiple of pad[a] (null = 1; App uses 32,4,1 as App.cpp:193). Solid nodes
+ * next to air carry the material of the triangle crossing the link to their first air
+ * neighbour (-x,+x,-y,+y,-z,+z). dims (nullable) receives X,Y,Z; labels (nullable, X*Y*Z
+ * bytes, x fastest) the label grid (0 air, m+1 solid of material m). */
+int pfdtd_voxelize(pfdtd_ctx *ctx, const float *vertices, int64_t n_vertices,
+                   const uint32_t *indices, int64_t n_triangles, const uint8_t *tri_material,
+                   float dx, const int32_t *pad, int64_t *dims, uint8_t *labels);
+/* Grid dims pfdtd_voxelize would produce (host only, no context). */
+int pfdtd_voxel_grid(const float *vertices, int64_t n_vertices, float dx, const int32_t *pad,
+                     int64_t *dims);
 /* Device -> host copy of the classified bytes (bit-exactness checks). */
 int pfdtd_get_domain(pfdtd_ctx *ctx, uint8_t *pos, uint8_t *mat);
 int pfdtd_get_dims(pfdtd_ctx *ctx, int64_t *X, int64_t *Y, int64_t *Z);

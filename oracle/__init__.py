@@ -72,6 +72,10 @@ def lib() -> C.CDLL:
         L.orc_f32_to_bf16.argtypes = [C.c_float]
         L.orc_bf16_to_f32.restype = C.c_float
         L.orc_bf16_to_f32.argtypes = [C.c_uint16]
+        L.orc_voxel_grid.restype = C.c_int
+        L.orc_voxel_grid.argtypes = [_p, _i64, C.c_float, _p, _p]
+        L.orc_voxelize.restype = C.c_int
+        L.orc_voxelize.argtypes = [_p, _i64, _p, _i64, _p, C.c_float, _p, _p, _p]
         _lib = L
     return _lib
 
@@ -81,6 +85,33 @@ def _ptr(a):
         return None
     assert a.flags["C_CONTIGUOUS"]
     return C.c_void_p(a.ctypes.data)
+
+
+def voxel_grid(vertices, dx: float, pad=(1, 1, 1)) -> tuple[int, int, int]:
+    """(X, Y, Z) of the voxel grid of a vertex array ([n][3] float32)."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    pad_a = np.asarray(pad, np.int32)
+    dims = np.zeros(3, np.int64)
+    rc = lib().orc_voxel_grid(_ptr(v), len(v), dx, _ptr(pad_a), _ptr(dims))
+    if rc:
+        raise ValueError(f"orc_voxel_grid failed ({rc})")
+    return tuple(int(d) for d in dims)
+
+
+def voxelize(vertices, indices, dx: float, tri_material=None, pad=(1, 1, 1)) -> np.ndarray:
+    """Label grid (Z, Y, X) of a closed triangle mesh: 0 air (inside), m+1 solid of material m."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    m = None if tri_material is None else np.ascontiguousarray(tri_material, np.uint8)
+    X, Y, Z = voxel_grid(v, dx, pad)
+    lab = np.empty((Z, Y, X), np.uint8)
+    pad_a = np.asarray(pad, np.int32)
+    dims = np.zeros(3, np.int64)
+    rc = lib().orc_voxelize(_ptr(v), len(v), _ptr(t), len(t), _ptr(m), dx, _ptr(pad_a),
+                            _ptr(dims), _ptr(lab))
+    if rc:
+        raise ValueError(f"orc_voxelize failed ({rc})")
+    return lab
 
 
 def classify(labels: np.ndarray):

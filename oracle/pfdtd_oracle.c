@@ -743,3 +743,165 @@ int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y
   }
   return -1;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Triangle-mesh solid voxelization (definition in pfdtd_oracle.h; no reference source)    */
+/* ------------------------------------------------------------------------------------ */
+
+#define VOX_SUB 4096
+#define VOX_MAX_DIM 32768
+#define VOX_NONE 0xFFFFFFFFu
+
+/* Fixed-point vertices (q may be NULL) and per-axis maximum. */
+static int vox_fixed(const float *v, int64_t nv, float dx, int32_t *q, int64_t ext[3]) {
+  if (!(dx > 0.f) || !isfinite(dx) || nv <= 0 || !v) return -1;
+  float mn[3];
+  for (int a = 0; a < 3; ++a) mn[a] = v[a];
+  for (int64_t i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float x = v[3 * i + a];
+      if (!isfinite(x)) return -1;
+      if (x < mn[a]) mn[a] = x;
+    }
+  ext[0] = ext[1] = ext[2] = 0;
+  const double lim = (double)(VOX_MAX_DIM - 3) * VOX_SUB;
+  for (int64_t i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float r = v[3 * i + a] - mn[a];
+      const double s = ((double)r * (double)VOX_SUB) / (double)dx;
+      if (!(s <= lim)) return -1;
+      const long long k = llrint(s);
+      if (q) q[3 * i + a] = (int32_t)k;
+      if (k > ext[a]) ext[a] = k;
+    }
+  return 0;
+}
+
+int orc_voxel_grid(const float *vertices, int64_t n_vertices, float dx, const int32_t pad[3],
+                   int64_t dims[3]) {
+  int64_t ext[3];
+  if (vox_fixed(vertices, n_vertices, dx, NULL, ext)) return -1;
+  for (int a = 0; a < 3; ++a) {
+    const int64_t p = (pad && pad[a] > 0) ? pad[a] : 1;
+    int64_t n = ext[a] / VOX_SUB + 3;
+    n = (n + p - 1) / p * p;
+    if (n > VOX_MAX_DIM) return -1;
+    dims[a] = n;
+  }
+  return 0;
+}
+
+static int64_t vox_edge(int64_t ux, int64_t uy, int64_t vx, int64_t vy, int64_t px, int64_t py) {
+  return (vx - ux) * (py - uy) - (vy - uy) * (px - ux);
+}
+
+/* a point on an edge belongs to the triangle on the edge's left (CCW) side iff the edge
+ * direction (ex, ey) points down, or left-to-right when horizontal: the minimum-side edges
+ * of a CCW triangle, so an axis-aligned square [0,L)^2 owns the columns of [0,L)^2 */
+static int vox_incl(int64_t w, int64_t ex, int64_t ey) {
+  return w > 0 || (w == 0 && (ey < 0 || (ey == 0 && ex > 0)));
+}
+
+static void vox_axis(const int32_t *q, const uint32_t *tri, int64_t nt, const int64_t d[3],
+                     int axis, uint8_t *flip, uint32_t *link) {
+  const int au = (axis + 1) % 3, av = (axis + 2) % 3;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int32_t *A = q + 3 * (int64_t)tri[3 * t], *B = q + 3 * (int64_t)tri[3 * t + 1],
+                  *C = q + 3 * (int64_t)tri[3 * t + 2];
+    const int64_t ax = A[au], ay = A[av], bx = B[au], by = B[av], cx = C[au], cy = C[av];
+    const int64_t area = vox_edge(ax, ay, bx, by, cx, cy);
+    if (area == 0) continue;
+    const int64_t s = area > 0 ? 1 : -1;
+    int64_t umin = ax, umax = ax, vmin = ay, vmax = ay;
+    if (bx < umin) umin = bx;
+    if (cx < umin) umin = cx;
+    if (bx > umax) umax = bx;
+    if (cx > umax) umax = cx;
+    if (by < vmin) vmin = by;
+    if (cy < vmin) vmin = cy;
+    if (by > vmax) vmax = by;
+    if (cy > vmax) vmax = cy;
+    int64_t u0 = (umin + VOX_SUB - 1) / VOX_SUB + 1, u1 = umax / VOX_SUB + 1;
+    int64_t v0 = (vmin + VOX_SUB - 1) / VOX_SUB + 1, v1 = vmax / VOX_SUB + 1;
+    if (u1 > d[au] - 1) u1 = d[au] - 1;
+    if (v1 > d[av] - 1) v1 = d[av] - 1;
+    for (int64_t nv_ = v0; nv_ <= v1; ++nv_)
+      for (int64_t nu = u0; nu <= u1; ++nu) {
+        const int64_t px = (nu - 1) * VOX_SUB, py = (nv_ - 1) * VOX_SUB;
+        const int64_t wa = s * vox_edge(bx, by, cx, cy, px, py);
+        const int64_t wb = s * vox_edge(cx, cy, ax, ay, px, py);
+        const int64_t wc = s * vox_edge(ax, ay, bx, by, px, py);
+        if (!vox_incl(wa, s * (cx - bx), s * (cy - by)) ||
+            !vox_incl(wb, s * (ax - cx), s * (ay - cy)) ||
+            !vox_incl(wc, s * (bx - ax), s * (by - ay)))
+          continue;
+        const double W = (double)(s * area);
+        const double num = (double)wb * (double)(B[axis] - A[axis]) +
+                           (double)wc * (double)(C[axis] - A[axis]);
+        const double h = (double)A[axis] + num / W;
+        int64_t n0 = (int64_t)ceil(h / (double)VOX_SUB) + 1;
+        if (n0 < 1) n0 = 1;
+        if (n0 >= d[axis]) continue;
+        int64_t c[3];
+        c[axis] = n0;
+        c[au] = nu;
+        c[av] = nv_;
+        const int64_t idx = (c[2] * d[1] + c[1]) * d[0] + c[0];
+        if (flip) flip[idx] ^= 1;
+        if ((uint32_t)t < link[idx]) link[idx] = (uint32_t)t;
+      }
+  }
+}
+
+int orc_voxelize(const float *vertices, int64_t n_vertices, const uint32_t *tri,
+                 int64_t n_tri, const uint8_t *tmat, float dx, const int32_t pad[3],
+                 int64_t dims[3], uint8_t *labels) {
+  if (orc_voxel_grid(vertices, n_vertices, dx, pad, dims)) return -1;
+  for (int64_t i = 0; i < 3 * n_tri; ++i)
+    if ((int64_t)tri[i] >= n_vertices) return -2;
+  if (tmat)
+    for (int64_t t = 0; t < n_tri; ++t)
+      if (tmat[t] > 254) return -3;
+  int64_t ext[3];
+  int32_t *q = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)n_vertices);
+  vox_fixed(vertices, n_vertices, dx, q, ext);
+  const int64_t X = dims[0], Y = dims[1], Z = dims[2], N = X * Y * Z;
+  uint8_t *air = (uint8_t *)calloc((size_t)N, 1);
+  uint32_t *link = (uint32_t *)malloc(sizeof(uint32_t) * 3 * (size_t)N);
+  memset(link, 0xFF, sizeof(uint32_t) * 3 * (size_t)N);
+  for (int axis = 0; axis < 3; ++axis)
+    vox_axis(q, tri, n_tri, dims, axis, axis == 0 ? air : NULL, link + axis * N);
+  for (int64_t r = 0; r < Y * Z; ++r) { /* prefix parity along each x row */
+    uint8_t p = 0;
+    for (int64_t i = 0; i < X; ++i) {
+      p ^= air[r * X + i];
+      air[r * X + i] = p;
+    }
+  }
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t y = 0; y < Y; ++y)
+      for (int64_t x = 0; x < X; ++x) {
+        const int64_t idx = (z * Y + y) * X + x;
+        if (air[idx]) {
+          labels[idx] = 0;
+          continue;
+        }
+        /* neighbour offsets and the node holding the link: -x,+x,-y,+y,-z,+z */
+        const int ok[6] = {x > 0, x + 1 < X, y > 0, y + 1 < Y, z > 0, z + 1 < Z};
+        const int64_t nb[6] = {idx - 1, idx + 1, idx - X, idx + X, idx - X * Y, idx + X * Y};
+        const int64_t lk[6] = {idx, idx + 1, N + idx, N + idx + X, 2 * N + idx,
+                               2 * N + idx + X * Y};
+        uint8_t lab = 1;
+        for (int k = 0; k < 6; ++k) {
+          if (!ok[k] || !air[nb[k]]) continue;
+          const uint32_t t = link[lk[k]];
+          if (t != VOX_NONE) lab = (uint8_t)(1 + (tmat ? tmat[t] : 0));
+          break;
+        }
+        labels[idx] = lab;
+      }
+  free(air);
+  free(link);
+  free(q);
+  return 0;
+}

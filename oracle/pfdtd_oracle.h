@@ -159,6 +159,44 @@ int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
 int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y, int64_t Z,
                         int64_t slice, int orientation, float *out_p, uint8_t *out_pos);
 
+/* Triangle-mesh solid voxelization into a label grid (0 = air inside the mesh, m+1 = solid
+ * voxel of material m), the input of orc_classify_labels / orc_classify_labels_kow.
+ *
+ * The reference delegates this to the external hakarlss/Voxelizer library (call site
+ * voxelizeGeometry, src/kernels/voxelizationUtils.cu:58-131; App::initializeMesh,
+ * src/App.cpp:178-187; build README.md:24), which is not vendored and has no pinned version
+ * in the reference, so there is no reference source to follow. README.md:100 states its
+ * method: intersection tests decide inside/outside, so the mesh must be watertight. This
+ * function states the definition the HIP voxelizer implements:
+ *  - frame: vertices minus their per-axis minimum (float subtraction, as
+ *    GeometryHandler::initialize, src/base/GeometryHandler.cpp:64-73), then fixed point
+ *    q = llrint(v * 4096 / dx) (double arithmetic);
+ *  - node (i,j,k) sits at ((i-1)dx, (j-1)dx, (k-1)dx): the +1 padding offset of
+ *    Position::getElementIdx (src/base/SrcRec.cpp:26-34); grid = floor(extent/dx) + 3 nodes
+ *    per axis, rounded up to a multiple of pad[a] (the (32,4,1) of App.cpp:193 / padWithZeros,
+ *    src/kernels/cudaMesh.cu:253-326, added at the high end);
+ *  - inside (air) = odd number of crossings of the x-ray through the node centre, counting a
+ *    triangle at column (y,z) when the column point lies inside its yz projection under a
+ *    top-left rule (exact integer edge functions), the crossing at
+ *    x* = xA + (wB (xB-xA) + wC (xC-xA)) / W (double, no contraction) and flipping nodes
+ *    with x >= x*;
+ *  - solid nodes take material 1 + tmat[t] of the triangle t crossing the link to their first
+ *    air face neighbour (order -x,+x,-y,+y,-z,+z; links rasterised per axis in the same way;
+ *    the lowest triangle index when several cross one link), 1 when no air neighbour or no
+ *    recorded crossing.
+ * Returns 0, -1 (bad dx / non-finite vertex / grid above 32768 per axis), -2 (triangle index
+ * out of range), -3 (material above 254).
+ * Pinned by (tests/test_oracle_voxel.py): closed-form box rooms (air counts and per-face
+ * materials, walls on and between node planes), nested solids, brute-force half-space
+ * point-in-convex-polyhedron tests for rotated boxes and tetrahedra, the enclosed volume of
+ * the reference's own meshes (python/Data/box.json, larun_hytti.json), and invariance under
+ * triangle order, vertex rotation, winding and translation. */
+int orc_voxel_grid(const float *vertices, int64_t n_vertices, float dx, const int32_t pad[3],
+                   int64_t dims[3]);
+int orc_voxelize(const float *vertices, int64_t n_vertices, const uint32_t *tri,
+                 int64_t n_tri, const uint8_t *tmat, float dx, const int32_t pad[3],
+                 int64_t dims[3], uint8_t *labels);
+
 /* Storage rounding helpers (round to nearest even), exposed for tests. */
 uint16_t orc_f32_to_f16(float f);
 float orc_f16_to_f32(uint16_t h);

@@ -31,7 +31,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-LIB_SOURCES = ["pfdtd_kernels.hip", "pfdtd_fused.hip", "pfdtd_capi.hip", "pfdtd_pool.hip"]
+LIB_SOURCES = ["pfdtd_kernels.hip", "pfdtd_fused.hip", "pfdtd_capi.hip", "pfdtd_pool.hip",
+               "pfdtd_voxel.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -330,12 +331,55 @@ void App::ensureContext() {
 void App::initializeGeometry(const unsigned int *indices, const float *vertices,
                              unsigned int number_of_indices, unsigned int number_of_vertices) {
   m_geometry.initialize(indices, vertices, number_of_indices, number_of_vertices);
+  geometry_domain_ = true;
+  has_domain_ = false;
+  labels_.clear();
 }
 
+// FileReader::readVTK, src/io/FileReader.cpp:41-89: among the first 20 lines, "POINTS n ..."
+// is followed by 3n coordinates in inches, converted to metres and rounded to
+// truncate_vertices_to = 0.01 m (ROUND(v*0.0254/t)*t in float, geomMath.h:126-128), and
+// "POLYGONS n size" by n rows "3 i1 i2 i3".
 void App::initializeGeometryFromFile(const std::string &path) {
-  log_line("initializeGeometryFromFile",
-           ("VTK geometry input is not part of this library (" + path + ")").c_str());
-  throw(-1);
+  std::ifstream in(path);
+  if (!in.good()) {
+    log_line("initializeGeometryFromFile", ("invalid file: " + path).c_str());
+    throw(-1);
+  }
+  const float trunc = 0.01f;
+  std::vector<float> vertices;
+  std::vector<unsigned int> indices;
+  std::string line;
+  for (int count = 0; count < 20 && std::getline(in, line); ++count) {
+    std::istringstream ss(line);
+    std::string key;
+    ss >> key;
+    if (key == "POINTS") {
+      unsigned int n = 0;
+      ss >> n;
+      vertices.assign((size_t)n * 3, 0.f);
+      for (size_t i = 0; i < vertices.size() && in; ++i) {
+        float v = 0.f;
+        in >> v;
+        v = std::floor((v * 0.0254f) / trunc + 0.5f);
+        vertices[i] = v * trunc;
+      }
+    } else if (key == "POLYGONS") {
+      unsigned int n = 0;
+      ss >> n;
+      indices.assign((size_t)n * 3, 0u);
+      for (size_t i = 0; i < indices.size() && in; i += 3) {
+        unsigned int k = 0;
+        in >> k >> indices[i] >> indices[i + 1] >> indices[i + 2];
+      }
+    }
+  }
+  if (!in && !in.eof()) {
+    log_line("initializeGeometryFromFile", ("malformed file: " + path).c_str());
+    throw(-1);
+  }
+  initializeGeometry(indices.data(), vertices.data(), (unsigned int)indices.size(),
+                     (unsigned int)vertices.size());
 }
 
 void App::initializeVoxels(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y,
@@ -345,6 +389,7 @@ void App::initializeVoxels(const uint8_t *pos, const uint8_t *mat, int64_t X, in
   check(pfdtd_set_update_type(ctx_, (int)m_parameters.getUpdateType()), "initializeVoxels");
   check(pfdtd_set_domain(ctx_, pos, mat, X, Y, Z), "initializeVoxels");
   labels_.clear();
+  geometry_domain_ = false;
   domain_type_ = (int)m_parameters.getUpdateType();
   uint64_t air = 0, bnd = 0;
   check(pfdtd_get_counts(ctx_, &air, &bnd), "initializeVoxels");
@@ -359,6 +404,7 @@ void App::initializeVoxelLabels(const uint8_t *labels, int64_t X, int64_t Y, int
   // kept so that a later setUpdateType can reclassify (the reference classifies for the
   // update type when the mesh is initialised, cudaMesh.cu:66-74)
   labels_.assign(labels, labels + (size_t)(X * Y * Z));
+  geometry_domain_ = false;
   labels_dims_[0] = X;
   labels_dims_[1] = Y;
   labels_dims_[2] = Z;
@@ -373,13 +419,36 @@ void App::initializeVoxelLabels(const uint8_t *labels, int64_t X, int64_t Y, int
 // parameters. The voxel domain is already resident.
 void App::initializeMesh() {
   ensureContext();
+  const int type = (int)m_parameters.getUpdateType();
+  if (geometry_domain_) {
+    // voxelizeGeometry + setupMesh (App.cpp:178-211): voxelize at dx, pad to (32,4,1)
+    const unsigned int n_tri = m_geometry.getNumberOfTriangles();
+    if (n_tri == 0 || m_materials.getNumberOfSurfaces() < n_tri) {
+      log_line("initializeMesh", "geometry needs a material index per triangle "
+                                 "(addSurfaceMaterials / setUniformMaterial)");
+      throw(-1);
+    }
+    std::vector<uint8_t> tmat(n_tri);
+    for (unsigned int t = 0; t < n_tri; ++t) tmat[t] = (uint8_t)m_materials.getMaterialIdxAt(t);
+    const int32_t pad[3] = {32, 4, 1};
+    int64_t dims[3] = {0, 0, 0};
+    check(pfdtd_set_update_type(ctx_, type), "initializeMesh");
+    check(pfdtd_voxelize(ctx_, m_geometry.vertices().data(),
+                         (int64_t)m_geometry.getNumberOfVertices(), m_geometry.indices().data(),
+                         (int64_t)n_tri, tmat.data(), m_parameters.getDx(), pad, dims, nullptr),
+          "initializeMesh");
+    uint64_t air = 0, bnd = 0;
+    check(pfdtd_get_counts(ctx_, &air, &bnd), "initializeMesh");
+    m_mesh.set(dims[0], dims[1], dims[2], air, bnd);
+    domain_type_ = type;
+    has_domain_ = true;
+  }
   if (!has_domain_) {
     log_line("initializeMesh",
-             "no voxel domain: call initializeVoxels/initializeVoxelLabels (triangle "
-             "voxelization is not part of this library)");
+             "no domain: call initializeGeometry/initializeGeometryFromFile or "
+             "initializeVoxels/initializeVoxelLabels");
     throw(-1);
   }
-  const int type = (int)m_parameters.getUpdateType();
   if ((type == SRL) != (domain_type_ == SRL)) {
     // the boundary formulation changed since the domain was classified
     if (labels_.empty()) {

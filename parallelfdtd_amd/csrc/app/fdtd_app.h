@@ -9,9 +9,10 @@
 // Errors follow the reference: device/runtime failures and unusable inputs throw int -1
 // (cudasafe, src/kernels/cudaUtils.h:47-51); out-of-range indices throw std::out_of_range.
 //
-// The domain comes in as voxels (initializeVoxels / initializeVoxelLabels); the triangle
-// voxelizer of the reference is a separate component (not part of this library), so
-// runSimulation without a voxel domain throws -1 with a message.
+// The domain comes in as a closed triangle mesh (initializeGeometry / initializeGeometryPy /
+// initializeGeometryFromFile, voxelized on the device by pfdtd_voxelize at every
+// initializeMesh, as the reference voxelizes in App::initializeMesh) or directly as voxels
+// (initializeVoxels / initializeVoxelLabels); runSimulation without either throws -1.
 #pragma once
 
 #include <cmath>
@@ -202,6 +203,7 @@ class GeometryHandler {
   unsigned int getNumberOfVertices() const { return (unsigned int)(vertices_.size() / 3); }
   const std::vector<float> &vertices() const { return vertices_; }
   const std::vector<unsigned int> &indices() const { return indices_; }
+  const std::vector<std::pair<std::string, std::vector<int>>> &layers() const { return layers_; }
 
  private:
   std::vector<float> vertices_;
@@ -256,7 +258,9 @@ class App {
   void initializeDevices();
   void initializeGeometry(const unsigned int *indices, const float *vertices,
                           unsigned int number_of_indices, unsigned int number_of_vertices);
-  void initializeGeometryFromFile(const std::string &path);  // VTK: not part of this library
+  // Legacy-VTK polydata (FileReader::readVTK, src/io/FileReader.cpp:41-89); throws -1 on a
+  // file that cannot be read (App.cpp:133-139).
+  void initializeGeometryFromFile(const std::string &path);
   void setupDefaultCallbacks();
   // Voxel domain: pre-classified position/material bytes (layout (z*Y+y)*X+x) ...
   void initializeVoxels(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z);
@@ -340,6 +344,7 @@ class App {
   pfdtd_ctx *ctx_ = nullptr;
   int device_ = 0;
   bool has_domain_ = false;
+  bool geometry_domain_ = false;  // the domain is voxelized from m_geometry at initializeMesh
   std::vector<uint8_t> labels_;  // label grid of initializeVoxelLabels (reclassification)
   int64_t labels_dims_[3] = {0, 0, 0};
   int domain_type_ = 0;          // update type the resident domain was classified for

@@ -41,7 +41,8 @@ PYBIND11_MODULE(libPyFDTD, m) {
       // initializeGeometryPy / setLayerIndicesPy, AppPy.cpp:24-50
       .def("initializeGeometryPy",
            [](FDTD::App &a, const std::vector<unsigned int> &idx, const std::vector<float> &vert) {
-             a.m_geometry.initialize(idx, vert);
+             a.initializeGeometry(idx.data(), vert.data(), (unsigned int)idx.size(),
+                                  (unsigned int)vert.size());
            })
       .def("setLayerIndices",
            [](FDTD::App &a, const std::vector<int> &idx, const std::string &name) {

@@ -647,6 +647,30 @@ int pfdtd_set_domain(pfdtd_ctx *c, const uint8_t *pos, const uint8_t *mat, int64
   return finish_domain(c);
 }
 
+// Classifies a device-resident label grid into the context's domain bytes for the current
+// boundary family (alloc_domain done by the caller).
+static int classify_device_labels(pfdtd_ctx *c, const uint8_t *d_lab) {
+  hipError_t e = hipSuccess;
+  unsigned *d_bad = nullptr;
+  c->degenerate = 0;
+  if (c->scheme == 2) {
+    e = hipMalloc((void **)&d_bad, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(unsigned), c->stream);
+    if (e == hipSuccess)
+      e = launch_classify_labels_centred(d_lab, c->d_pos, c->d_mat, (int)c->X, (int)c->Y,
+                                         (int)c->Z, d_bad, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&c->degenerate, d_bad, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+  } else {
+    e = launch_classify_labels(d_lab, c->d_pos, c->d_mat, (int)c->X, (int)c->Y, (int)c->Z,
+                               c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (d_bad) (void)hipFree(d_bad);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "label classification: %s", hipGetErrorString(e));
+  return finish_domain(c);
+}
+
 int pfdtd_set_domain_labels(pfdtd_ctx *c, const uint8_t *labels, int64_t X, int64_t Y,
                             int64_t Z) {
   int rc = check_ctx(c);
@@ -660,24 +684,68 @@ int pfdtd_set_domain_labels(pfdtd_ctx *c, const uint8_t *labels, int64_t X, int6
     return fail(PFDTD_E_NOMEM, "cannot allocate the label grid");
   }
   hipError_t e = hipMemcpy(d_lab, labels, (size_t)c->N, hipMemcpyHostToDevice);
-  unsigned *d_bad = nullptr;
-  c->degenerate = 0;
-  if (e == hipSuccess && c->scheme == 2) {
-    e = hipMalloc((void **)&d_bad, sizeof(unsigned));
-    if (e == hipSuccess) e = hipMemsetAsync(d_bad, 0, sizeof(unsigned), c->stream);
-    if (e == hipSuccess)
-      e = launch_classify_labels_centred(d_lab, c->d_pos, c->d_mat, (int)X, (int)Y, (int)Z, d_bad,
-                                         c->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(&c->degenerate, d_bad, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
-  } else if (e == hipSuccess) {
-    e = launch_classify_labels(d_lab, c->d_pos, c->d_mat, (int)X, (int)Y, (int)Z, c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(d_lab);
+    return fail(PFDTD_E_HIP, "label upload: %s", hipGetErrorString(e));
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  rc = classify_device_labels(c, d_lab);
   (void)hipFree(d_lab);
-  if (d_bad) (void)hipFree(d_bad);
-  if (e != hipSuccess) return fail(PFDTD_E_HIP, "label classification: %s", hipGetErrorString(e));
-  return finish_domain(c);
+  return rc;
+}
+
+int pfdtd_voxel_grid(const float *vertices, int64_t n_vertices, float dx, const int32_t *pad,
+                     int64_t *dims) {
+  if (!vertices || !dims || n_vertices <= 0) return fail(PFDTD_E_ARG, "null or empty vertex array");
+  if (pfdtd::voxel_prepare(vertices, n_vertices, dx, pad, dims, nullptr))
+    return fail(PFDTD_E_ARG, "voxel grid: dx must be positive and finite, vertices finite, "
+                             "and the grid at most 32768 nodes per axis");
+  return PFDTD_OK;
+}
+
+int pfdtd_voxelize(pfdtd_ctx *c, const float *vertices, int64_t n_vertices,
+                   const uint32_t *indices, int64_t n_triangles, const uint8_t *tri_material,
+                   float dx, const int32_t *pad, int64_t *dims, uint8_t *labels) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!vertices || n_vertices <= 0) return fail(PFDTD_E_ARG, "null or empty vertex array");
+  if (!indices || n_triangles <= 0) return fail(PFDTD_E_ARG, "null or empty triangle array");
+  for (int64_t i = 0; i < 3 * n_triangles; ++i)
+    if ((int64_t)indices[i] >= n_vertices)
+      return fail(PFDTD_E_ARG, "triangle %lld names vertex %u of %lld", (long long)(i / 3),
+                  indices[i], (long long)n_vertices);
+  if (tri_material)
+    for (int64_t t = 0; t < n_triangles; ++t)
+      if (tri_material[t] > 254)
+        return fail(PFDTD_E_ARG, "triangle %lld: material %u above 254", (long long)t,
+                    tri_material[t]);
+  int64_t d[3];
+  std::vector<int32_t> q;
+  if (pfdtd::voxel_prepare(vertices, n_vertices, dx, pad, d, &q))
+    return fail(PFDTD_E_ARG, "voxel grid: dx must be positive and finite, vertices finite, "
+                             "and the grid at most 32768 nodes per axis");
+  rc = alloc_domain(c, d[0], d[1], d[2]);
+  if (rc) return rc;
+  uint8_t *d_lab = nullptr;
+  if (hipMalloc((void **)&d_lab, (size_t)c->N) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PFDTD_E_NOMEM, "cannot allocate the label grid");
+  }
+  hipError_t e = pfdtd::voxelize_labels(q, indices, n_triangles, tri_material, d, d_lab, c->stream);
+  if (e == hipSuccess && labels) e = hipMemcpy(labels, d_lab, (size_t)c->N, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    (void)hipFree(d_lab);
+    (void)hipGetLastError();
+    return fail(e == hipErrorOutOfMemory ? PFDTD_E_NOMEM : PFDTD_E_HIP, "voxelization: %s",
+                hipGetErrorString(e));
+  }
+  rc = classify_device_labels(c, d_lab);
+  (void)hipFree(d_lab);
+  if (rc == PFDTD_OK && dims) {
+    dims[0] = d[0];
+    dims[1] = d[1];
+    dims[2] = d[2];
+  }
+  return rc;
 }
 
 int pfdtd_set_domain_codes(pfdtd_ctx *c, const uint8_t *codes, const uint8_t *mat, int64_t X,

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace pfdtd {
 
 // Per-(K, material) update coefficients of the SRL_FORWARD rule
@@ -174,6 +176,15 @@ void build_centred_tables(const float *table20, int n_mat, unsigned octave, doub
 hipError_t launch_capture_slice(int precision, const void *field, const uint8_t *pos, int X,
                                 int Y, int Z, int64_t slice, int orientation, int scale_exp,
                                 float *out_p, uint8_t *out_pos, hipStream_t stream);
+// Triangle-mesh voxelization (pfdtd_voxel.hip): fixed-point vertices and grid dims of a mesh
+// (0, or -1 on a bad dx / non-finite vertex / grid above 32768 per axis), then the label grid
+// of the mesh written to d_labels (dims[0]*dims[1]*dims[2] bytes). Triangle indices must
+// already be checked against the vertex count.
+int voxel_prepare(const float *v, int64_t nv, float dx, const int32_t *pad, int64_t dims[3],
+                  std::vector<int32_t> *q);
+hipError_t voxelize_labels(const std::vector<int32_t> &q, const uint32_t *tri, int64_t n_tri,
+                           const uint8_t *tmat, const int64_t dims[3], uint8_t *d_labels,
+                           hipStream_t s);
 // Whole field to unscaled FP32.
 hipError_t launch_field_to_f32(int precision, const void *field, int64_t n, int scale_exp,
                                float *out, hipStream_t stream);

@@ -579,8 +579,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   if (!job_id) return pfail(PFDTD_E_ARG, "null job id output");
   rc = check_job(j);
   if (rc) return rc;
-  int resident = 0;
-  for (auto &kv : p->jobs) resident += 1;
+  const int resident = (int)p->jobs.size();
   if (resident >= p->max_jobs) return pfail(PFDTD_E_NOMEM, "all %d job slots in use", p->max_jobs);
   int slot = -1;
   for (int s = 0; s < kPoolMaxJobs; ++s)

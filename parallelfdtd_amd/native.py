@@ -52,6 +52,8 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_domain": [_p, _p, _p, _i64, _i64, _i64],
     "pfdtd_set_domain_labels": [_p, _p, _i64, _i64, _i64],
     "pfdtd_set_domain_codes": [_p, _p, _p, _i64, _i64, _i64],
+    "pfdtd_voxelize": [_p, _p, _i64, _p, _i64, _p, C.c_float, _p, _p, _p],
+    "pfdtd_voxel_grid": [_p, _i64, C.c_float, _p, _p],
     "pfdtd_get_domain": [_p, _p, _p],
     "pfdtd_get_dims": [_p, _pi64, _pi64, _pi64],
     "pfdtd_get_counts": [_p, _pu64, _pu64],

@@ -14,6 +14,16 @@ import numpy as np
 from . import native as N
 
 
+def voxel_grid(vertices: np.ndarray, dx: float, pad=(1, 1, 1)) -> tuple[int, int, int]:
+    """(X, Y, Z) of the voxel grid pfdtd_voxelize builds for these vertices."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    pad_a = np.ascontiguousarray(pad, np.int32)
+    dims = np.zeros(3, np.int64)
+    N.check(N.lib().pfdtd_voxel_grid(N.ptr(v), len(v), C.c_float(dx), N.ptr(pad_a), N.ptr(dims)),
+            "pfdtd_voxel_grid")
+    return tuple(int(d) for d in dims)
+
+
 class Solver:
     """One device context: domain, materials, sources, receivers, fields, responses."""
 
@@ -63,6 +73,26 @@ class Solver:
         Z, Y, X = labels.shape
         self._c("pfdtd_set_domain_labels", N.ptr(labels), X, Y, Z)
         self.dims = (X, Y, Z)
+
+    def voxelize(self, vertices: np.ndarray, indices: np.ndarray, dx: float,
+                 tri_material: np.ndarray | None = None, pad=(1, 1, 1),
+                 want_labels: bool = False) -> np.ndarray | None:
+        """Voxelize a closed triangle mesh (vertices [n][3] metres, indices [m][3], one material
+        index per triangle) on the GPU and classify it as the domain; returns the (Z, Y, X)
+        label grid when asked."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        m = None if tri_material is None else np.ascontiguousarray(tri_material, np.uint8)
+        pad_a = np.ascontiguousarray(pad, np.int32)
+        X, Y, Z = voxel_grid(v, dx, pad)
+        lab = np.empty((Z, Y, X), np.uint8) if want_labels else None
+        dims = np.zeros(3, np.int64)
+        self._c("pfdtd_voxelize", N.ptr(v), len(v), N.ptr(t), len(t),
+                None if m is None else N.ptr(m), C.c_float(dx), N.ptr(pad_a), N.ptr(dims),
+                None if lab is None else N.ptr(lab))
+        assert tuple(int(d) for d in dims) == (X, Y, Z)
+        self.dims = (X, Y, Z)
+        return lab
 
     def set_domain_codes(self, codes: np.ndarray, mat: np.ndarray) -> None:
         """(Z, Y, X) voxelizer node codes 0..27 and material bytes (toBilbao, or toKowalczyk for

@@ -204,3 +204,9 @@ def test_transparent_source_through_app(gpu):
     ref = _oracle_responses(app, lab, betas, n_steps, "f32")
     assert np.array_equal(app.getResponse(0), ref[0])
     app.close()
+
+
+def test_geometry_file_errors():
+    app = pf.App()
+    with pytest.raises(RuntimeError):
+        app.initializeGeometryFromFile("/nonexistent/room.vtk")

@@ -82,3 +82,15 @@ def test_pool_job_bytes_without_device():
     j.precision = native.F64
     assert lib.pfdtd_pool_job_bytes(ctypes.byref(j), ctypes.byref(b)) == native.E_ARG
     assert lib.pfdtd_pool_destroy(None) == native.OK
+
+
+def test_voxel_grid_without_device():
+    """pfdtd_voxel_grid is host-only: dims of the reference's box room (python/Data/box.json,
+    a 1 m cube) at dx = 0.1 are floor(1/0.1) + 3 = 13 per axis, padded to (32, 4, 1)."""
+    from parallelfdtd_amd.solver import voxel_grid
+    from parallelfdtd_amd import geometry as G
+    b = G.box_mesh((0, 0, 0), (1, 1, 1))
+    assert voxel_grid(b.vertices, 0.1) == (13, 13, 13)
+    assert voxel_grid(b.vertices, 0.1, (32, 4, 1)) == (32, 16, 13)
+    with pytest.raises(native.PfdtdError):
+        voxel_grid(b.vertices, 0.0)
