@@ -24,13 +24,14 @@
 //    material of the crossing on the link to its first air neighbour (-x,+x,-y,+y,-z,+z),
 //    1 when it has no air neighbour or the link recorded no crossing.
 //
-// Kernels (one launch each, grid-stride loops, 64-bit indices):
+// Kernels (one launch each, 64-bit indices):
 //  vox_raster_kernel  -- one work item per (triangle, candidate column) pair, found by binary
 //                        search over the per-triangle running column count; parity events
 //                        are XORed into a bit grid (x rows of 32-bit words), link crossings
 //                        min-reduced with 32-bit atomics;
 //  vox_prefix_kernel  -- one thread per x row: prefix XOR of the events -> air bits;
-//  vox_label_kernel   -- one thread per node: air bit, first air neighbour, link material.
+//  vox_label_kernel   -- one thread per node (3D grid over x-row blocks, y, z): air bit, first
+//                        air neighbour, link material.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -138,35 +139,36 @@ __global__ __launch_bounds__(256) void vox_prefix_kernel(uint32_t *bits, int64_t
   }
 }
 
+// grid (ceil(X/256), Y, Z): one thread per node of an x row, no index divisions
 __global__ __launch_bounds__(256) void vox_label_kernel(const uint32_t *bits, const uint32_t *link,
                                                         const uint8_t *tmat, int64_t X, int64_t Y,
                                                         int64_t Z, int64_t words, uint8_t *labels) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= X) return;
+  const int64_t y = blockIdx.y, z = blockIdx.z;
   const int64_t N = X * Y * Z;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < N; idx += stride) {
-    const int64_t x = idx % X, yz = idx / X, y = yz % Y, z = yz / Y;
-    auto air = [&](int64_t xx, int64_t yy, int64_t zz) -> bool {
-      return (bits[(zz * Y + yy) * words + (xx >> 5)] >> (xx & 31)) & 1u;
-    };
-    if (air(x, y, z)) {
-      labels[idx] = 0;
-      continue;
-    }
-    uint8_t lab = 1;
-    const bool ok[6] = {x > 0, x + 1 < X, y > 0, y + 1 < Y, z > 0, z + 1 < Z};
-    const int64_t nx[6] = {x - 1, x + 1, x, x, x, x};
-    const int64_t ny[6] = {y, y, y - 1, y + 1, y, y};
-    const int64_t nz[6] = {z, z, z, z, z - 1, z + 1};
-    const int64_t lk[6] = {idx, idx + 1, N + idx, N + idx + X, 2 * N + idx, 2 * N + idx + X * Y};
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      if (!ok[k] || !air(nx[k], ny[k], nz[k])) continue;
-      const uint32_t t = link[lk[k]];
-      if (t != kVoxNone) lab = (uint8_t)(1 + (tmat ? tmat[t] : 0));
-      break;
-    }
-    labels[idx] = lab;
+  const int64_t idx = (z * Y + y) * X + x;
+  auto air = [&](int64_t xx, int64_t yy, int64_t zz) -> bool {
+    return (bits[(zz * Y + yy) * words + (xx >> 5)] >> (xx & 31)) & 1u;
+  };
+  if (air(x, y, z)) {
+    labels[idx] = 0;
+    return;
   }
+  uint8_t lab = 1;
+  const bool ok[6] = {x > 0, x + 1 < X, y > 0, y + 1 < Y, z > 0, z + 1 < Z};
+  const int64_t nx[6] = {x - 1, x + 1, x, x, x, x};
+  const int64_t ny[6] = {y, y, y - 1, y + 1, y, y};
+  const int64_t nz[6] = {z, z, z, z, z - 1, z + 1};
+  const int64_t lk[6] = {idx, idx + 1, N + idx, N + idx + X, 2 * N + idx, 2 * N + idx + X * Y};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    if (!ok[k] || !air(nx[k], ny[k], nz[k])) continue;
+    const uint32_t t = link[lk[k]];
+    if (t != kVoxNone) lab = (uint8_t)(1 + (tmat ? tmat[t] : 0));
+    break;
+  }
+  labels[idx] = lab;
 }
 
 inline unsigned grid_for(int64_t n) {
@@ -292,7 +294,8 @@ hipError_t voxelize_labels(const std::vector<int32_t> &q, const uint32_t *tri, i
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
-    vox_label_kernel<<<grid_for(N), 256, 0, s>>>(d_bits, d_link, d_tmat, X, Y, Z, words, d_labels);
+    const dim3 grid((unsigned)((X + 255) / 256), (unsigned)Y, (unsigned)Z);
+    vox_label_kernel<<<grid, 256, 0, s>>>(d_bits, d_link, d_tmat, X, Y, Z, words, d_labels);
     e = hipGetLastError();
   }
   const hipError_t es = hipStreamSynchronize(s);
