@@ -494,6 +494,9 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
 bool fused_eligible(const pfdtd_ctx *c) {
   if (c->scheme != 0 || c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
     return false;
+  // the two-level kernel addresses each z-slice through a buffer descriptor (32-bit range)
+  const uint64_t esz = c->precision == PFDTD_F64 ? 8 : (c->precision == PFDTD_F32 ? 4 : 2);
+  if ((uint64_t)c->X * (uint64_t)c->Y * esz >= (1ull << 32)) return false;
   for (int s = 0; s < c->n_src; ++s) {
     const int32_t *q = &c->src_xyz[3 * s];
     if (lin_idx(c, q) >= 0 && (q[2] == 0 || q[2] == c->Z - 1)) return false;

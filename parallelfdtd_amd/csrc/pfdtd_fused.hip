@@ -4,12 +4,15 @@
 // 18 bytes of HBM traffic per two FP32 node-updates instead of 28. The four field buffers are
 // distinct, so blocks never read what another block writes and need no synchronisation.
 //
-// Tiling: each wavefront owns a 60-column x 8-row x-y tile and one z-chunk. Its 64 lanes cover
-// the 60 owned columns plus two halo columns on each side. P^{n+1} is computed on the owned tile
-// plus a one-voxel ring (columns lanes 1..62, 10 rows), from P^n on a two-voxel ring (12 rows);
-// P^{n+2} is then computed on the owned tile. x-neighbours come from cross-lane shuffles, y- and
-// z-neighbours from registers: the wave marches z with P^n of three slices and P^{n+1} of three
-// slices in registers, the second time level trailing the first by one slice.
+// Tiling: each wavefront owns a 60-column x TY-row x-y tile (TY = 6 for FP32, 4 for FP64, 10 for
+// 16-bit storage) and one z-chunk. Its 64 lanes cover the 60 owned columns plus two halo
+// columns on each side. P^{n+1} is computed on the owned tile plus a one-voxel ring (lanes
+// 1..62, TY+2 rows), from P^n on a two-voxel ring (TY+4 rows); P^{n+2} is then computed on the
+// owned tile. x-neighbours come from cross-lane DPP shifts, y- and z-neighbours from registers:
+// the wave marches z with P^n of three slices and P^{n+1} of three slices in registers, the
+// second time level trailing the first by one slice. Every field and byte access of a slice is a
+// buffer instruction (descriptor of the slice in scalar registers, loop-invariant 32-bit lane
+// offsets), and all loads of a slice are issued before any is consumed.
 //
 // Source injection of step n+1 and the receiver gather of step n happen inside the kernel on
 // P^{n+1}, in the reference's order (gather before the next injection; sources in index order),
@@ -66,22 +69,54 @@ __device__ __forceinline__ double lane_prev(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// Buffer access to one z-slice: the descriptor (base = the slice, range = its bytes) lives in
+// scalar registers and every lane passes a loop-invariant 32-bit byte offset, so no per-slice
+// 64-bit addresses are formed in vector registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(const void *p, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)nbytes,
+                                           0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+  if constexpr (sizeof(T) == 1) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b8(r, (int)e, 0, 0));
+  } else if constexpr (sizeof(T) == 2) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, (int)(e * 2u), 0, 0));
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 4u), 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(e * 8u), 0, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, uint32_t e, T v) {
+  if constexpr (sizeof(T) == 2) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, (int)(e * 2u), 0, 0);
+  } else if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)(e * 4u), 0, 0);
+  } else {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)(e * 8u), 0, 0);
+  }
+}
+
 // Packed position|material<<8 of rows 2i and 2i+1 in one register.
 template <int NB>
 __device__ __forceinline__ unsigned pm_of(const uint32_t (&b)[NB], int j) {
   return (b[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
 }
 
-// Owned rows per wave tile. 2-byte fields: 8 (the fewer halo rows per owned row win).
-// FP32 and FP64: 4, which keeps the kernel at <= 128 VGPRs (four waves per SIMD); measured on
-// the 2048x1536x768 room, FP32 with 8 rows (179 VGPRs, two waves) was 9 % slower.
+// Owned rows per wave tile (must be even: the byte planes of two rows share a register).
+// Measured on the 2048x1536x768 room: 2-byte fields 10 (166 VGPRs, three waves per SIMD; 8 %
+// faster than 8 rows). FP32 6, the most that keeps the kernel within 128 VGPRs (four waves per
+// SIMD; 3 % faster than 4 rows). FP64: 4.
 template <typename S>
 struct FusedTile {
-  static constexpr int TY = 8;
+  static constexpr int TY = 10;
 };
 template <>
 struct FusedTile<float> {
-  static constexpr int TY = 4;
+  static constexpr int TY = 6;
 };
 template <>
 struct FusedTile<double> {
@@ -92,6 +127,7 @@ template <typename S, typename CT, typename C>
 __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S, CT, C> a, int X, int Y, int Z, int zc) {
 #pragma clang fp contract(off)
   constexpr int kFTY = FusedTile<S>::TY;
+  static_assert(kFTY % 2 == 0, "byte planes of two rows share one register");
   constexpr int NA = kFTY + 4;   // P^n rows per wave tile (2-voxel y halo)
   constexpr int NQ = kFTY + 2;   // P^{n+1} rows (1-voxel y halo)
   constexpr int NB = NQ / 2;     // packed byte registers
@@ -128,6 +164,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   if (yt >= Y) return;  // no barrier follows
 
   const size_t XY = (size_t)X * (size_t)Y;
+  const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^32
   const int xc = min(max(x, 0), X - 1);
   // Rows r = 0..11 of the P^n block are y = yt - 2 + r (clamped to the grid for loads).
   uint32_t ro[NA];
@@ -179,26 +216,31 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
                   uint32_t (&bp)[NB], uint32_t (&bc)[NB], int k) {
     const bool upd1 = k >= 1 && k <= Z - 2;
     if (k + 1 <= Z - 1) {
-      const S *An = gA + (size_t)(k + 1) * XY;
+      const auto rA = slice_rsrc(gA + (size_t)(k + 1) * XY, sbytes);
 #pragma unroll
-      for (int r = 0; r < NA; ++r) ap[r] = Fmt<S>::val(gld(An, ro[r]));
+      for (int r = 0; r < NA; ++r) ap[r] = Fmt<S>::val(bld<S>(rA, ro[r]));
     } else {
 #pragma unroll
       for (int r = 0; r < NA; ++r) ap[r] = (C)0;
     }
     C past[NQ];
     if (upd1) {
-      const S *Bk = gB + (size_t)k * XY;
-      const uint8_t *pk = gpos + (size_t)k * XY;
-      const uint8_t *mk = gmat + (size_t)k * XY;
+      const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
+      const auto rP = slice_rsrc(gpos + (size_t)k * XY, (uint32_t)XY);
+      const auto rM = slice_rsrc(gmat + (size_t)k * XY, (uint32_t)XY);
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) past[j] = Fmt<S>::val(gld(Bk, ro[j + 1]));
+      for (int j = 0; j < NQ; ++j) past[j] = Fmt<S>::val(bld<S>(rB, ro[j + 1]));
+      uint32_t pv[NQ], mv[NQ];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        const uint32_t r0 = ro[2 * i + 1], r1 = ro[2 * i + 2];
-        bc[i] = (uint32_t)gld(pk, r0) | ((uint32_t)gld(mk, r0) << 8) |
-                ((uint32_t)gld(pk, r1) << 16) | ((uint32_t)gld(mk, r1) << 24);
+      for (int j = 0; j < NQ; ++j) {
+        pv[j] = bld<uint8_t>(rP, ro[j + 1]);
+        mv[j] = bld<uint8_t>(rM, ro[j + 1]);
       }
+      // every load of the slice is in flight before any result is combined
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        bc[i] = pv[2 * i] | (mv[2 * i] << 8) | (pv[2 * i + 1] << 16) | (mv[2 * i + 1] << 24);
     } else {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) past[j] = (C)0;
@@ -263,15 +305,15 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
 
     // ---- store the owned part of P^{n+1}[k]
     if (k >= z0 && k < z1) {
-      S *Ck = gC + (size_t)k * XY;
+      const auto rC = slice_rsrc(gC + (size_t)k * XY, sbytes);
 #pragma unroll
       for (int o = 0; o < kFTY; ++o)
-        if ((smask >> o) & 1u) gst(Ck, ro[o + 2], Fmt<S>::raw(q2[o + 1]));
+        if ((smask >> o) & 1u) bst<S>(rC, ro[o + 2], Fmt<S>::raw(q2[o + 1]));
     }
 
     // ---- second time level: P^{n+2}[k-1] on the owned rows
     if (k - 1 >= z0) {
-      S *Dk = gD + (size_t)(k - 1) * XY;
+      const auto rD = slice_rsrc(gD + (size_t)(k - 1) * XY, sbytes);
       auto level2 = [&](auto table) {
 #pragma unroll
         for (int o = 0; o < kFTY; ++o) {
@@ -283,7 +325,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
           s = s + lane_prev(q1[j]);          // x-1
           C v = node(table, pm_of(bp, j), q1[j], s, am[o + 2]);
           v = ((imask >> (o + 2)) & 1u) ? v : (C)0;
-          if ((smask >> o) & 1u) gst(Dk, ro[o + 2], Fmt<S>::raw(v));
+          if ((smask >> o) & 1u) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
         }
       };
       if (any_nonair(bp))
@@ -299,13 +341,13 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   const int k0 = z0 - 1;
   const int k1 = z1;
   {
-    const S *Ak = gA + (size_t)k0 * XY;
+    const auto rK = slice_rsrc(gA + (size_t)k0 * XY, sbytes);
 #pragma unroll
-    for (int r = 0; r < NA; ++r) A1[r] = Fmt<S>::val(gld(Ak, ro[r]));
+    for (int r = 0; r < NA; ++r) A1[r] = Fmt<S>::val(bld<S>(rK, ro[r]));
     if (k0 >= 1) {
-      const S *Am = gA + (size_t)(k0 - 1) * XY;
+      const auto rKm = slice_rsrc(gA + (size_t)(k0 - 1) * XY, sbytes);
 #pragma unroll
-      for (int r = 0; r < NA; ++r) A0[r] = Fmt<S>::val(gld(Am, ro[r]));
+      for (int r = 0; r < NA; ++r) A0[r] = Fmt<S>::val(bld<S>(rKm, ro[r]));
     } else {
 #pragma unroll
       for (int r = 0; r < NA; ++r) A0[r] = (C)0;
