@@ -514,6 +514,12 @@ Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C la
   a.Cn = (S *)c->d_Pp2;
   a.Dn = (S *)c->d_P2;
   a.coef = coef;
+  a.ncol = 1;  // every material byte present in the domain (all validated) is below ncol
+  for (int m = 255; m > 0; --m)
+    if ((c->scan.mat_present[m >> 5] >> (m & 31)) & 1u) {
+      a.ncol = m + 1;
+      break;
+    }
   a.air = air;
   a.air_uniform = c->air_uniform;
   a.lam2 = lam2;

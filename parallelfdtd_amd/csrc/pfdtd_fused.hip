@@ -154,6 +154,15 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);  // owned slices [z0, z1)
 
+  // The (K, material) coefficient rows the domain can address, staged in LDS: gathers from
+  // LDS neither go to L2 nor share the vector-memory counter with the field loads.
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  typedef typename CoefOf<C>::T CE;
+  CE *s_coef = reinterpret_cast<CE *>(s_dyn);
+  const unsigned ncol = (unsigned)a.ncol;
+  for (unsigned i = threadIdx.x; i < kCoefRows * ncol; i += 64 * kFWaves)
+    s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
+
   // Sources that land on this wave's P^{n+1} region (owned tile + 1-voxel ring, slices
   // z0-1 .. z1) and receivers on its owned tile.
   const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kFO + 1, yt - 1, yt + kFTY + 1,
@@ -183,14 +192,13 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   const double inv_scale = ldexp(1.0, -a.scale_exp);
   const C scale = (C)ldexp(1.0, a.scale_exp);
 
-  const typename CoefOf<C>::T *__restrict__ coef = a.coef;
   const typename CoefOf<C>::T air = a.air;
   const C lam2 = a.lam2;
   // TABLE: gather the (K, material) entry; otherwise every voxel of the stage is air.
   auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
     const unsigned pb = pm & 0xFFu;
     typename CoefOf<C>::T e = air;
-    if (decltype(table)::value) e = coef[min(pb & 0x7Fu, 7u) * 256u + (pm >> 8)];
+    if (decltype(table)::value) e = s_coef[min(pb & 0x7Fu, 7u) * ncol + (pm >> 8)];
     const C aa = (C)(pb >> 7) * e.inv;
     return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
   };
@@ -389,7 +397,8 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision) {
 template <typename S, typename CT, typename C>
 static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a,
                                hipStream_t stream) {
-  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C>), g.grid, g.block, 0, stream, a, g.X, g.Y,
+  const size_t lds = sizeof(CT) * kCoefRows * (size_t)a.ncol;
+  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C>), g.grid, g.block, lds, stream, a, g.X, g.Y,
                      g.Z, g.zc);
   return hipGetLastError();
 }

@@ -92,6 +92,7 @@ struct Fused2Args {
   S *Cn;
   S *Dn;
   const CT *coef;
+  int ncol;  // material columns staged in LDS: highest material byte in the domain + 1
   CT air;
   int air_uniform;
   C lam2;
