@@ -466,24 +466,33 @@ InjectArgs inject_args(const pfdtd_ctx *c, int gather_step, int inject_step) {
   return a;
 }
 
+// Material columns the update kernels stage in LDS: the highest material byte present in the
+// domain + 1 (every present byte has been checked against the table, check_runnable).
+int coef_cols(const pfdtd_ctx *c) {
+  for (int m = 255; m > 0; --m)
+    if ((c->scan.mat_present[m >> 5] >> (m & 31)) & 1u) return m + 1;
+  return 1;
+}
+
 hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
+  const int ncol = coef_cols(c);
   switch (c->precision) {
   case PFDTD_F32:
     return launch_update_f32(g, UpdateArgs<float, CoefF, float>{c->d_pos, c->d_mat,
-                             (const float *)c->d_P, (float *)c->d_Pp, c->d_coef_f, c->air_f,
-                             c->air_uniform, c->lam2_f, c->scheme}, c->stream);
+                             (const float *)c->d_P, (float *)c->d_Pp, c->d_coef_f, ncol,
+                             c->air_f, c->air_uniform, c->lam2_f, c->scheme}, c->stream);
   case PFDTD_F64:
     return launch_update_f64(g, UpdateArgs<double, CoefD, double>{c->d_pos, c->d_mat,
-                             (const double *)c->d_P, (double *)c->d_Pp, c->d_coef_d, c->air_d,
-                             c->air_uniform, c->lam2_d, c->scheme}, c->stream);
+                             (const double *)c->d_P, (double *)c->d_Pp, c->d_coef_d, ncol,
+                             c->air_d, c->air_uniform, c->lam2_d, c->scheme}, c->stream);
   case PFDTD_F16:
     return launch_update_f16(g, UpdateArgs<h16, CoefF, float>{c->d_pos, c->d_mat,
-                             (const h16 *)c->d_P, (h16 *)c->d_Pp, c->d_coef_f, c->air_f,
-                             c->air_uniform, c->lam2_f, c->scheme}, c->stream);
+                             (const h16 *)c->d_P, (h16 *)c->d_Pp, c->d_coef_f, ncol,
+                             c->air_f, c->air_uniform, c->lam2_f, c->scheme}, c->stream);
   case PFDTD_BF16:
     return launch_update_bf16(g, UpdateArgs<b16, CoefF, float>{c->d_pos, c->d_mat,
-                              (const b16 *)c->d_P, (b16 *)c->d_Pp, c->d_coef_f, c->air_f,
-                              c->air_uniform, c->lam2_f, c->scheme}, c->stream);
+                              (const b16 *)c->d_P, (b16 *)c->d_Pp, c->d_coef_f, ncol,
+                              c->air_f, c->air_uniform, c->lam2_f, c->scheme}, c->stream);
   default:
     return hipErrorInvalidValue;
   }
@@ -514,12 +523,7 @@ Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C la
   a.Cn = (S *)c->d_Pp2;
   a.Dn = (S *)c->d_P2;
   a.coef = coef;
-  a.ncol = 1;  // every material byte present in the domain (all validated) is below ncol
-  for (int m = 255; m > 0; --m)
-    if ((c->scan.mat_present[m >> 5] >> (m & 31)) & 1u) {
-      a.ncol = m + 1;
-      break;
-    }
+  a.ncol = coef_cols(c);
   a.air = air;
   a.air_uniform = c->air_uniform;
   a.lam2 = lam2;

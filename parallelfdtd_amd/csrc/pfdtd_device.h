@@ -113,15 +113,29 @@ struct CoefOf<double> {
   typedef CoefD T;
 };
 
+// Copies the coefficient rows a domain can address -- ncol material columns of each of the
+// kCoefRows rows of the [row][256] device table -- into the block's LDS (row stride ncol), then
+// waits for the whole block. Gathers from LDS neither go to L2 nor share the vector-memory
+// counter with the field loads. Every thread of the block must call it.
+template <typename CE>
+__device__ __forceinline__ void stage_coef(CE *__restrict__ s, const CE *__restrict__ g,
+                                           unsigned ncol) {
+  for (unsigned i = threadIdx.x; i < kCoefRows * ncol; i += blockDim.x)
+    s[i] = g[(i / ncol) * 256u + i % ncol];
+  __syncthreads();
+}
+
 // One SRL_FORWARD node update (kernels3d.cu:508-528) from the neighbour sum s (already summed
 // in the order z+1, z-1, y+1, y-1, x+1, x-1), the node's value pc and past value past.
+// coef is the block's LDS copy of the coefficient table (stage_coef), cs material columns
+// per row.
 template <typename C>
 __device__ __forceinline__ C srl_node(unsigned pb, unsigned mb, C pc, C s, C past,
-                                      const typename CoefOf<C>::T *__restrict__ coef,
+                                      const typename CoefOf<C>::T *__restrict__ coef, unsigned cs,
                                       const typename CoefOf<C>::T &air, int air_uniform, C lam2) {
 #pragma clang fp contract(off)
   typename CoefOf<C>::T e = air;
-  if (pb != 0x86u || !air_uniform) e = coef[min(pb & 0x7Fu, 7u) * 256u + mb];
+  if (pb != 0x86u || !air_uniform) e = coef[min(pb & 0x7Fu, 7u) * cs + mb];
   const C a = (C)(pb >> 7) * e.inv;
   return a * ((e.tk * pc + lam2 * s) - e.omb * past);
 }
@@ -154,11 +168,11 @@ template <typename C>
 __device__ __forceinline__ C centred_node(unsigned pb, unsigned mb, C pc, C zp, C zm, C yp, C ym,
                                           C xp, C xm, C past,
                                           const typename CoefOf<C>::T *__restrict__ coef,
-                                          const typename CoefOf<C>::T &air, C lam2) {
+                                          unsigned cs, const typename CoefOf<C>::T &air, C lam2) {
 #pragma clang fp contract(off)
   const unsigned dxb = pb & 1u, dyb = (pb >> 1) & 1u, dzb = (pb >> 2) & 1u;
   typename CoefOf<C>::T e = air;
-  if (pb != 0x80u) e = coef[(dxb + dyb + dzb) * 256u + mb];
+  if (pb != 0x80u) e = coef[(dxb + dyb + dzb) * cs + mb];
   const C sx = (pb & 0x10u) ? xp : xm;  // p_x[sign_x]
   const C sy = (pb & 0x20u) ? yp : ym;  // p_y[sign_y]
   const C sz = (pb & 0x40u) ? zm : zp;  // p_z[sign_z]: p_z[1] is z-1
@@ -185,8 +199,8 @@ template <typename S, int RY, int TW = kTX, int SCHEME = 0>
 __device__ __forceinline__ void srl_tile(
     const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
     S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
-    typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
-    int X, int Y, int Z, int zc, int bx, int by, int bz) {
+    unsigned cs, typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform,
+    typename Fmt<S>::C lam2, int X, int Y, int Z, int zc, int bx, int by, int bz) {
 #pragma clang fp contract(off)
   typedef typename Fmt<S>::C C;
   // TW threads along x; 256 / TW row bands of RY rows each stacked in y
@@ -242,7 +256,7 @@ __device__ __forceinline__ void srl_tile(
       C ret;
       if (SCHEME == 2) {
         ret = centred_node<C>(pb[r], mb[r], pc[r], pn[r], pm[r], yp, ym, xp[r], xm[r], past[r],
-                              coef, air, lam2);
+                              coef, cs, air, lam2);
       } else {
         // Sum order z+1, z-1, y+1, y-1, x+1, x-1 (kernels3d.cu:516-525).
         C s = pn[r] + pm[r];
@@ -251,7 +265,7 @@ __device__ __forceinline__ void srl_tile(
         s = s + xp[r];
         s = s + xm[r];
         typename CoefOf<C>::T e = air;
-        if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * 256u + mb[r]];
+        if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * cs + mb[r]];
         const C a = (C)(pb[r] >> 7) * e.inv;
         ret = a * ((e.tk * pc[r] + lam2 * s) - e.omb * past[r]);
       }

@@ -63,6 +63,7 @@ struct UpdateArgs {
   const S *P;
   S *Pp;
   const CT *coef;
+  int ncol;  // material columns staged in LDS: highest material byte in the domain + 1
   CT air;
   int air_uniform;
   C lam2;

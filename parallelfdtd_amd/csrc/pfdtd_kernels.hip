@@ -21,10 +21,14 @@ template <typename S, int RY, int SCHEME = 0>
 __global__ __launch_bounds__(kTX) void srl_update_kernel(
     const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
     S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
-    typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
-    int X, int Y, int Z, int zc) {
-  srl_tile<S, RY, kTX, SCHEME>(pos, mat, P, Pp, coef, air, air_uniform, lam2, X, Y, Z, zc,
-                               (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+    unsigned ncol, typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform,
+    typename Fmt<S>::C lam2, int X, int Y, int Z, int zc) {
+  typedef typename CoefOf<typename Fmt<S>::C>::T CE;
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  CE *s_coef = reinterpret_cast<CE *>(s_dyn);
+  stage_coef(s_coef, coef, ncol);
+  srl_tile<S, RY, kTX, SCHEME>(pos, mat, P, Pp, s_coef, ncol, air, air_uniform, lam2, X, Y, Z,
+                               zc, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -40,10 +44,14 @@ template <typename S, int RY>
 __global__ __launch_bounds__(kTX) void srl_update_batch_kernel(
     const uint8_t *__restrict__ pos, const uint8_t *__restrict__ mat, const S *__restrict__ P,
     S *__restrict__ Pp, const typename CoefOf<typename Fmt<S>::C>::T *__restrict__ coef,
-    typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform, typename Fmt<S>::C lam2,
-    int X, int Y, int Z, int zc, int lgB) {
+    unsigned ncol, typename CoefOf<typename Fmt<S>::C>::T air, int air_uniform,
+    typename Fmt<S>::C lam2, int X, int Y, int Z, int zc, int lgB) {
 #pragma clang fp contract(off)
   typedef typename Fmt<S>::C C;
+  typedef typename CoefOf<C>::T CE;
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  CE *s_coef = reinterpret_cast<CE *>(s_dyn);
+  stage_coef(s_coef, coef, ncol);
   const int B = 1 << lgB;
   const int b = threadIdx.x & (B - 1);
   const int x = blockIdx.x * (kTX >> lgB) + (int)(threadIdx.x >> lgB);
@@ -101,7 +109,8 @@ __global__ __launch_bounds__(kTX) void srl_update_batch_kernel(
       s = s + ym;
       s = s + xp[r];
       s = s + xm[r];
-      const C ret = srl_node<C>(pb[r], mb[r], pc[r], s, past[r], coef, air, air_uniform, lam2);
+      const C ret =
+          srl_node<C>(pb[r], mb[r], pc[r], s, past[r], s_coef, ncol, air, air_uniform, lam2);
       if (live[r]) Fmt<S>::st(Pp, (base + row[r]) * B + b, upd[r] ? ret : (C)0);
     }
 #pragma unroll
@@ -140,15 +149,18 @@ UpdateGeom plan_update(int X, int Y, int Z, int ry, int zc) {
 template <typename S, typename CT, typename C>
 static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> &a,
                                 hipStream_t stream) {
-  if ((a.scheme != 0 && a.scheme != 2) || (a.scheme == 2 && g.lgB > 0))
+  if ((a.scheme != 0 && a.scheme != 2) || (a.scheme == 2 && g.lgB > 0) || a.ncol < 1 ||
+      a.ncol > 256)
     return hipErrorInvalidValue;
+  const unsigned ncol = (unsigned)a.ncol;
+  const size_t lds = sizeof(CT) * kCoefRows * (size_t)ncol;
   if (g.lgB > 0) {
     switch (g.ry) {
 #define PFDTD_RYB_CASE(R)                                                                    \
   case R:                                                                                    \
-    hipLaunchKernelGGL((srl_update_batch_kernel<S, R>), g.grid, g.block, 0, stream, a.pos,   \
-                       a.mat, a.P, a.Pp, a.coef, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z, \
-                       g.zc, g.lgB);                                                         \
+    hipLaunchKernelGGL((srl_update_batch_kernel<S, R>), g.grid, g.block, lds, stream, a.pos, \
+                       a.mat, a.P, a.Pp, a.coef, ncol, a.air, a.air_uniform, a.lam2, g.X, g.Y, \
+                       g.Z, g.zc, g.lgB);                                                    \
     break;
       PFDTD_RYB_CASE(1)
       PFDTD_RYB_CASE(2)
@@ -164,9 +176,9 @@ static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> 
     switch (g.ry) {
 #define PFDTD_RYC_CASE(R)                                                                    \
   case R:                                                                                    \
-    hipLaunchKernelGGL((srl_update_kernel<S, R, 2>), g.grid, g.block, 0, stream, a.pos,      \
-                       a.mat, a.P, a.Pp, a.coef, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z, \
-                       g.zc);                                                                \
+    hipLaunchKernelGGL((srl_update_kernel<S, R, 2>), g.grid, g.block, lds, stream, a.pos,    \
+                       a.mat, a.P, a.Pp, a.coef, ncol, a.air, a.air_uniform, a.lam2, g.X, g.Y, \
+                       g.Z, g.zc);                                                           \
     break;
       PFDTD_RYC_CASE(1)
       PFDTD_RYC_CASE(2)
@@ -181,8 +193,9 @@ static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> 
   switch (g.ry) {
 #define PFDTD_RY_CASE(R)                                                                     \
   case R:                                                                                    \
-    hipLaunchKernelGGL((srl_update_kernel<S, R>), g.grid, g.block, 0, stream, a.pos, a.mat,  \
-                       a.P, a.Pp, a.coef, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z, g.zc); \
+    hipLaunchKernelGGL((srl_update_kernel<S, R>), g.grid, g.block, lds, stream, a.pos, a.mat,\
+                       a.P, a.Pp, a.coef, ncol, a.air, a.air_uniform, a.lam2, g.X, g.Y, g.Z,  \
+                       g.zc);                                                                \
     break;
     PFDTD_RY_CASE(1)
     PFDTD_RY_CASE(2)

@@ -41,6 +41,7 @@ struct PoolJobDev {
   void *A;  // field buffers: parity 0 -> A current, B past; parity 1 -> swapped
   void *B;
   const CoefF *coef;
+  int ncol;  // material columns of the table the job's domain addresses (its n_mat)
   CoefF air;
   int air_uniform;
   float lam2;
@@ -64,6 +65,7 @@ struct PoolState {
 // Launch table, passed by value: the jobs of one launch and their cumulative tile counts.
 struct PoolLaunch {
   int n;
+  int ncol_max;  // largest material column count of the launch's jobs (LDS bytes per block)
   int slot[kPoolMaxJobs];
   int tile_end[kPoolMaxJobs];
 };
@@ -92,8 +94,12 @@ __global__ __launch_bounds__(kTX) void pool_update_kernel(const PoolJobDev *__re
   const int bz = r / J.gy;
   const S *P = (const S *)(st.parity ? J.B : J.A);
   S *Pp = (S *)(st.parity ? J.A : J.B);
-  srl_tile<S, RY, kPoolTW>(J.pos, J.mat, P, Pp, J.coef, J.air, J.air_uniform, J.lam2, J.X, J.Y,
-                           J.Z, zc, bx, by, bz);
+  // every thread of the block serves the same job, so the whole block reaches the barrier
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
+  stage_coef(s_coef, J.coef, (unsigned)J.ncol);
+  srl_tile<S, RY, kPoolTW>(J.pos, J.mat, P, Pp, s_coef, (unsigned)J.ncol, J.air, J.air_uniform,
+                           J.lam2, J.X, J.Y, J.Z, zc, bx, by, bz);
 }
 
 template <typename S>
@@ -129,15 +135,16 @@ __global__ __launch_bounds__(256) void pool_inject_kernel(const PoolJobDev *__re
 template <typename S>
 hipError_t launch_pool_update_t(int ry, int tiles, const PoolJobDev *jobs, const PoolState *st,
                                 const PoolLaunch &L, int zc, hipStream_t stream) {
+  const size_t lds = sizeof(CoefF) * kCoefRows * (size_t)L.ncol_max;
   switch (ry) {
   case 1:
-    hipLaunchKernelGGL((pool_update_kernel<S, 1>), dim3(tiles), dim3(kTX), 0, stream, jobs, st, L, zc);
+    hipLaunchKernelGGL((pool_update_kernel<S, 1>), dim3(tiles), dim3(kTX), lds, stream, jobs, st, L, zc);
     break;
   case 2:
-    hipLaunchKernelGGL((pool_update_kernel<S, 2>), dim3(tiles), dim3(kTX), 0, stream, jobs, st, L, zc);
+    hipLaunchKernelGGL((pool_update_kernel<S, 2>), dim3(tiles), dim3(kTX), lds, stream, jobs, st, L, zc);
     break;
   case 4:
-    hipLaunchKernelGGL((pool_update_kernel<S, 4>), dim3(tiles), dim3(kTX), 0, stream, jobs, st, L, zc);
+    hipLaunchKernelGGL((pool_update_kernel<S, 4>), dim3(tiles), dim3(kTX), lds, stream, jobs, st, L, zc);
     break;
   default:
     return hipErrorInvalidValue;
@@ -243,6 +250,7 @@ struct PoolJobHost {
   int n_steps = 0, n_rec = 0;
   int64_t voxels = 0;
   int tiles = 0;
+  int ncol = 1;
   size_t off = 0, bytes = 0;
   int64_t steps_enqueued = 0;
   uint64_t final_seq = 0;
@@ -397,6 +405,7 @@ int rebuild_membership(pfdtd_pool *p) {
   p->members.clear();
   for (int q = 0; q < 3; ++q) {
     p->launch[q].n = 0;
+    p->launch[q].ncol_max = 1;
     p->tiles[q] = 0;
   }
   for (auto &kv : p->jobs) {
@@ -405,6 +414,7 @@ int rebuild_membership(pfdtd_pool *p) {
     const int q = j.precision == PFDTD_F32 ? 0 : (j.precision == PFDTD_F16 ? 1 : 2);
     PoolLaunch &L = p->launch[q];
     L.slot[L.n] = j.slot;
+    L.ncol_max = std::max(L.ncol_max, j.ncol);
     p->tiles[q] += j.tiles;
     L.tile_end[L.n] = p->tiles[q];
     ++L.n;
@@ -621,6 +631,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   d.A = base + L.A;
   d.B = base + L.B;
   d.coef = (const CoefF *)(base + L.coef);
+  d.ncol = j->n_mat;  // labels were checked to name materials 0..n_mat-1 only
   d.air = cf[6 * 256];
   d.air_uniform = air_uniform;
   d.lam2 = (float)(j->lambda * j->lambda);
@@ -711,6 +722,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   h.id = p->next_id++;
   h.slot = slot;
   h.precision = j->precision;
+  h.ncol = j->n_mat;
   h.n_steps = j->n_steps;
   h.n_rec = j->n_rec;
   h.voxels = N;
