@@ -170,7 +170,8 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kFO, yt, yt + kFTY, z0, z1,
                                 s_rec[wave], lane);
   __syncthreads();
-  if (yt >= Y) return;  // no barrier follows
+  // Waves whose tile lies past the grid stay in the z loop (every row masked) so that the
+  // block's waves can step z together.
 
   const size_t XY = (size_t)X * (size_t)Y;
   const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^32
@@ -222,6 +223,9 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   // index row-1); bp/bc: packed bytes at k-1 and k (rows 1..10).
   auto body = [&](C (&am)[NA], C (&ac)[NA], C (&ap)[NA], C (&q0)[NQ], C (&q1)[NQ], C (&q2)[NQ],
                   uint32_t (&bp)[NB], uint32_t (&bc)[NB], int k) {
+    // the block's four waves load overlapping halo rows: keeping them on the same slice lets
+    // the cache serve the overlap once
+    __builtin_amdgcn_s_barrier();
     const bool upd1 = k >= 1 && k <= Z - 2;
     if (k + 1 <= Z - 1) {
       const auto rA = slice_rsrc(gA + (size_t)(k + 1) * XY, sbytes);
