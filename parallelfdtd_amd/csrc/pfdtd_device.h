@@ -114,13 +114,13 @@ struct CoefOf<double> {
 };
 
 // Copies the coefficient rows a domain can address -- ncol material columns of each of the
-// kCoefRows rows of the [row][256] device table -- into the block's LDS (row stride ncol), then
-// waits for the whole block. Gathers from LDS neither go to L2 nor share the vector-memory
-// counter with the field loads. Every thread of the block must call it.
+// first kStageRows rows of the [row][256] device table -- into the block's LDS (row stride
+// ncol), then waits for the whole block. Gathers from LDS neither go to L2 nor share the
+// vector-memory counter with the field loads. Every thread of the block must call it.
 template <typename CE>
 __device__ __forceinline__ void stage_coef(CE *__restrict__ s, const CE *__restrict__ g,
                                            unsigned ncol) {
-  for (unsigned i = threadIdx.x; i < kCoefRows * ncol; i += blockDim.x)
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += blockDim.x)
     s[i] = g[(i / ncol) * 256u + i % ncol];
   __syncthreads();
 }
@@ -135,7 +135,7 @@ __device__ __forceinline__ C srl_node(unsigned pb, unsigned mb, C pc, C s, C pas
                                       const typename CoefOf<C>::T &air, int air_uniform, C lam2) {
 #pragma clang fp contract(off)
   typename CoefOf<C>::T e = air;
-  if (pb != 0x86u || !air_uniform) e = coef[min(pb & 0x7Fu, 7u) * cs + mb];
+  if (pb != 0x86u || !air_uniform) e = coef[min(pb & 0x7Fu, 6u) * cs + mb];
   const C a = (C)(pb >> 7) * e.inv;
   return a * ((e.tk * pc + lam2 * s) - e.omb * past);
 }
@@ -265,7 +265,7 @@ __device__ __forceinline__ void srl_tile(
         s = s + xp[r];
         s = s + xm[r];
         typename CoefOf<C>::T e = air;
-        if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 7u) * cs + mb[r]];
+        if (pb[r] != 0x86u || !air_uniform) e = coef[min(pb[r] & 0x7Fu, 6u) * cs + mb[r]];
         const C a = (C)(pb[r] >> 7) * e.inv;
         ret = a * ((e.tk * pc[r] + lam2 * s) - e.omb * past[r]);
       }

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   typedef typename CoefOf<C>::T CE;
   CE *s_coef = reinterpret_cast<CE *>(s_dyn);
   const unsigned ncol = (unsigned)a.ncol;
-  for (unsigned i = threadIdx.x; i < kCoefRows * ncol; i += 64 * kFWaves)
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * kFWaves)
     s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
 
   // Sources that land on this wave's P^{n+1} region (owned tile + 1-voxel ring, slices
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
     const unsigned pb = pm & 0xFFu;
     typename CoefOf<C>::T e = air;
-    if (decltype(table)::value) e = s_coef[min(pb & 0x7Fu, 7u) * ncol + (pm >> 8)];
+    if (decltype(table)::value) e = s_coef[min(pb & 0x7Fu, 6u) * ncol + (pm >> 8)];
     const C aa = (C)(pb >> 7) * e.inv;
     return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
   };
@@ -401,7 +401,7 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision) {
 template <typename S, typename CT, typename C>
 static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a,
                                hipStream_t stream) {
-  const size_t lds = sizeof(CT) * kCoefRows * (size_t)a.ncol;
+  const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
   hipLaunchKernelGGL((srl_update2_kernel<S, CT, C>), g.grid, g.block, lds, stream, a, g.X, g.Y,
                      g.Z, g.zc);
   return hipGetLastError();

@@ -22,6 +22,10 @@ struct CoefD {
 };
 constexpr int kCoefRows = 8;
 constexpr int kCoefEntries = kCoefRows * 256;
+// Rows the update kernels stage in LDS: K = 0..6 (forward) or D = 0..3 (centred). Inside voxels
+// never name K > 6 (the domain scan rejects them) and outside voxels are multiplied by 0, so
+// the row index is clamped to kStageRows - 1; FP64 tables of 256 materials stay at 56 KB.
+constexpr int kStageRows = 7;
 
 // Storage tags for the half-width field formats.
 struct h16 {

@@ -153,7 +153,7 @@ static hipError_t launch_update(const UpdateGeom &g, const UpdateArgs<S, CT, C> 
       a.ncol > 256)
     return hipErrorInvalidValue;
   const unsigned ncol = (unsigned)a.ncol;
-  const size_t lds = sizeof(CT) * kCoefRows * (size_t)ncol;
+  const size_t lds = sizeof(CT) * kStageRows * (size_t)ncol;
   if (g.lgB > 0) {
     switch (g.ry) {
 #define PFDTD_RYB_CASE(R)                                                                    \

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void pool_inject_kernel(const PoolJobDev *__re
 template <typename S>
 hipError_t launch_pool_update_t(int ry, int tiles, const PoolJobDev *jobs, const PoolState *st,
                                 const PoolLaunch &L, int zc, hipStream_t stream) {
-  const size_t lds = sizeof(CoefF) * kCoefRows * (size_t)L.ncol_max;
+  const size_t lds = sizeof(CoefF) * kStageRows * (size_t)L.ncol_max;
   switch (ry) {
   case 1:
     hipLaunchKernelGGL((pool_update_kernel<S, 1>), dim3(tiles), dim3(kTX), lds, stream, jobs, st, L, zc);
