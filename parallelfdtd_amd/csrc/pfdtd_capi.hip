@@ -68,6 +68,8 @@ struct pfdtd_ctx {
   // domain
   int64_t X = 0, Y = 0, Z = 0, N = 0;
   uint8_t *d_pos = nullptr, *d_mat = nullptr;
+  uint16_t *d_pm = nullptr;  // pos | mat << 8, built for the two-level update (pm_ok: current)
+  bool pm_ok = false;
   int64_t dom_cap = 0;  // voxels the position/material buffers hold
   bool has_domain = false;
   DomainScan scan{};
@@ -314,8 +316,10 @@ int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
   if (c->d_pos && c->dom_cap != N) {
     dfree(c->d_pos);
     dfree(c->d_mat);
+    dfree(c->d_pm);
     c->dom_cap = 0;
   }
+  c->pm_ok = false;
   // fields sized for the previous grid are dropped
   free_fields(c);
   if (!c->d_pos) {
@@ -516,8 +520,7 @@ bool fused_eligible(const pfdtd_ctx *c) {
 template <typename S, typename CT, typename C>
 Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step) {
   Fused2Args<S, CT, C> a;
-  a.pos = c->d_pos;
-  a.mat = c->d_mat;
+  a.pm = c->d_pm;
   a.A = (const S *)c->d_P;
   a.B = (const S *)c->d_Pp;
   a.Cn = (S *)c->d_Pp2;
@@ -627,6 +630,7 @@ int pfdtd_ctx_destroy(pfdtd_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dfree(c->d_pos);
   dfree(c->d_mat);
+  dfree(c->d_pm);
   dfree(c->d_scan);
   dfree(c->d_coef_f);
   dfree(c->d_coef_d);
@@ -1029,7 +1033,19 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   // step 0's source samples go into the (zero) current field before the first update
   if (n_steps > 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, 0), c->stream));
 
-  const bool fused = fused_eligible(c);
+  bool fused = fused_eligible(c);
+  if (fused && !c->pm_ok) {
+    // the two-level sweep reads position and material as one 16-bit plane (2 B per voxel);
+    // without room for it the run takes single steps
+    if (!c->d_pm && hipMalloc((void **)&c->d_pm, sizeof(uint16_t) * (size_t)c->N) != hipSuccess) {
+      (void)hipGetLastError();
+      c->d_pm = nullptr;
+      fused = false;
+    } else {
+      HIP_TRY(launch_pack_pm(c->d_pos, c->d_mat, c->d_pm, c->N, c->stream));
+      c->pm_ok = true;
+    }
+  }
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision);
   int done = 0;
   auto wall0 = std::chrono::steady_clock::now();

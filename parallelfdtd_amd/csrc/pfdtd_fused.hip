@@ -215,8 +215,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   const S *__restrict__ gB = a.B;
   S *__restrict__ gC = a.Cn;
   S *__restrict__ gD = a.Dn;
-  const uint8_t *__restrict__ gpos = a.pos;
-  const uint8_t *__restrict__ gmat = a.mat;
+  const uint16_t *__restrict__ gpm = a.pm;
 
   // One z-iteration: first level at slice k (rows 1..10), second level at slice k-1 (rows 2..9).
   // am/ac/ap: P^n at k-1, k, k+1 (12 rows); q0/q1/q2: P^{n+1} at k-2, k-1, k (rows 1..10 at
@@ -238,21 +237,16 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
     C past[NQ];
     if (upd1) {
       const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
-      const auto rP = slice_rsrc(gpos + (size_t)k * XY, (uint32_t)XY);
-      const auto rM = slice_rsrc(gmat + (size_t)k * XY, (uint32_t)XY);
+      const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
 #pragma unroll
       for (int j = 0; j < NQ; ++j) past[j] = Fmt<S>::val(bld<S>(rB, ro[j + 1]));
-      uint32_t pv[NQ], mv[NQ];
+      uint32_t pmv[NQ];
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) {
-        pv[j] = bld<uint8_t>(rP, ro[j + 1]);
-        mv[j] = bld<uint8_t>(rM, ro[j + 1]);
-      }
+      for (int j = 0; j < NQ; ++j) pmv[j] = bld<uint16_t>(rPM, ro[j + 1]);
       // every load of the slice is in flight before any result is combined
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
-        bc[i] = pv[2 * i] | (mv[2 * i] << 8) | (pv[2 * i + 1] << 16) | (mv[2 * i + 1] << 24);
+      for (int i = 0; i < NB; ++i) bc[i] = pmv[2 * i] | (pmv[2 * i + 1] << 16);
     } else {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) past[j] = (C)0;

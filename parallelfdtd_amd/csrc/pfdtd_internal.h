@@ -90,8 +90,7 @@ hipError_t launch_update_bf16(const UpdateGeom &g, const UpdateArgs<b16, CoefF, 
 // (-2^30, -2^30, -2^30).
 template <typename S, typename CT, typename C>
 struct Fused2Args {
-  const uint8_t *pos;
-  const uint8_t *mat;
+  const uint16_t *pm;  // position byte | material byte << 8 per voxel
   const S *A;
   const S *B;
   S *Cn;
@@ -165,6 +164,9 @@ void build_coef_tables(const float *table20, int n_mat, unsigned octave, double 
 hipError_t launch_classify_labels(const uint8_t *labels, uint8_t *pos, uint8_t *mat, int X,
                                   int Y, int Z, hipStream_t stream);
 hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStream_t stream);
+// pm[i] = pos[i] | mat[i] << 8 (the two-level update's packed byte plane).
+hipError_t launch_pack_pm(const uint8_t *pos, const uint8_t *mat, uint16_t *pm, int64_t n,
+                          hipStream_t stream);
 hipError_t launch_scan_domain(const uint8_t *pos, const uint8_t *mat, int X, int Y, int Z,
                               int scheme, DomainScan *d_out, hipStream_t stream);
 // Centred-scheme (update type SRL) classification of labels / voxelizer codes.

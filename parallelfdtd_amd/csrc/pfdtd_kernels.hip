@@ -479,6 +479,22 @@ hipError_t launch_codes_to_bilbao(uint8_t *pos, uint8_t *mat, int64_t n, hipStre
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void pack_pm_kernel(const uint8_t *__restrict__ pos,
+                                                      const uint8_t *__restrict__ mat,
+                                                      uint16_t *__restrict__ pm, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    pm[i] = (uint16_t)(pos[i] | ((unsigned)mat[i] << 8));
+}
+
+hipError_t launch_pack_pm(const uint8_t *pos, const uint8_t *mat, uint16_t *pm, int64_t n,
+                          hipStream_t stream) {
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+  hipLaunchKernelGGL(pack_pm_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, pos, mat, pm,
+                     n);
+  return hipGetLastError();
+}
+
 // Counts air/boundary voxels (calcBoundaries, cudaMesh.cu:500-514; air value 0x86 for the
 // forward scheme, 0x80 for the centred one, cudaMesh.cu:176-186) and checks the
 // preconditions the update relies on. Blocks stride over (y, z) rows, threads over x; each
