@@ -279,8 +279,9 @@ typedef struct pfdtd_job {
 /* arena_bytes <= 0: 7/8 of the device's free memory. */
 int pfdtd_pool_create(int device, int64_t arena_bytes, pfdtd_pool **out);
 int pfdtd_pool_destroy(pfdtd_pool *pool);
-/* Steps per bundle (default 32), hipGraph replay (1, default) or direct launches (0), update
- * tiling (rows 1/2/4, z-chunk; only while empty) and the resident-job limit (1..64). */
+/* Steps per bundle (default 32; rounded up to an even count, as jobs advance in rounds of two
+ * steps), hipGraph replay (1, default) or direct launches (0), single-step update tiling
+ * (rows 1/2/4, z-chunk; only while empty) and the resident-job limit (1..64). */
 int pfdtd_pool_set_options(pfdtd_pool *pool, int bundle_steps, int use_graph, int rows,
                            int zchunk, int max_jobs);
 /* Arena bytes a job occupies while resident. */

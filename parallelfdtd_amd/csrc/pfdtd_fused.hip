@@ -17,6 +17,7 @@
 // Source injection of step n+1 and the receiver gather of step n happen inside the kernel on
 // P^{n+1}, in the reference's order (gather before the next injection; sources in index order),
 // so every node's arithmetic -- and storage rounding -- equals two single steps bit for bit.
+#include "../../include/pfdtd.h"
 #include "pfdtd_device.h"
 
 #include <type_traits>
@@ -123,8 +124,22 @@ struct FusedTile<double> {
   static constexpr int TY = 4;
 };
 
+// Kernel arguments: one context's domain, or the two-level jobs of one serving-pool launch.
 template <typename S, typename CT, typename C>
-__global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S, CT, C> a, int X, int Y, int Z, int zc) {
+struct Fused2Ctx {
+  Fused2Args<S, CT, C> a;
+  int X, Y, Z, zc;
+};
+struct Fused2Pool {
+  const PoolJobDev *jobs;
+  const PoolState *state;
+  PoolLaunch L;
+  int zc;
+};
+
+template <typename S, typename CT, typename C, bool POOL>
+__global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
+    typename std::conditional<POOL, Fused2Pool, Fused2Ctx<S, CT, C>>::type p) {
 #pragma clang fp contract(off)
   constexpr int kFTY = FusedTile<S>::TY;
   static_assert(kFTY % 2 == 0, "byte planes of two rows share one register");
@@ -138,8 +153,15 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
   // XCD-aware tile order: consecutive block ids are dealt round-robin to the 8 XCDs (each with
   // its own L2); remapping them so that every XCD walks a contiguous run of x-fastest tiles keeps
   // the halo columns and rows that neighbouring tiles share in one L2 while both are resident.
-  int bxi, byi, bzi;
-  {
+  Fused2Args<S, CT, C> a;
+  int X, Y, Z, zc, bxi, byi, bzi;
+  bool two = true;  // second time level (a pool job with one step left takes only the first)
+  if constexpr (!POOL) {
+    a = p.a;
+    X = p.X;
+    Y = p.Y;
+    Z = p.Z;
+    zc = p.zc;
     const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
     const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
     const unsigned full = total & ~7u;
@@ -147,6 +169,58 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
     bxi = (int)(tile % nx);
     byi = (int)((tile / nx) % ny);
     bzi = (int)(tile / (nx * ny));
+  } else {
+    // the flattened tiles of every job of the launch; the block finds its job by a binary
+    // search over the launch table's cumulative tile counts
+    const unsigned total = gridDim.x, lin = blockIdx.x, full = total & ~7u;
+    const int tile = (int)(lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin);
+    int lo = 0, hi = p.L.n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (p.L.tile_end[mid] > tile)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    const PoolJobDev &J = p.jobs[p.L.slot[lo]];
+    const PoolState st = p.state[p.L.slot[lo]];
+    const int remaining = J.n_steps - st.step;
+    if (remaining <= 0) return;  // the whole block: no barrier has been reached
+    two = remaining >= 2;
+    int o1, o2;
+    pool_spare(st.cur, st.past, o1, o2);
+    a.pm = J.pm;
+    a.A = (const S *)J.F[st.cur];
+    a.B = (const S *)J.F[st.past];
+    a.Cn = (S *)J.F[o1];
+    a.Dn = (S *)J.F[o2];
+    a.coef = J.coef;
+    a.ncol = J.ncol;
+    a.air = J.air;
+    a.air_uniform = J.air_uniform;
+    a.lam2 = J.lam2;
+    a.src_xyz = J.src_xyz;
+    a.src_kind = J.src_kind;
+    a.n_src = J.n_src;
+    a.sig_f = J.sig;
+    a.sig_d = nullptr;
+    a.n_samples = J.n_samples;
+    a.inject_step = (J.n_src > 0 && st.step + 1 < J.n_steps) ? st.step + 1 : -1;
+    a.soft_overwrite = J.soft_overwrite;
+    a.scale_exp = J.scale_exp;
+    a.rec_xyz = J.rec_xyz;
+    a.n_rec = J.n_rec;
+    a.resp = J.resp;
+    a.resp_steps = J.n_steps;
+    a.gather_step = st.step;
+    X = J.X;
+    Y = J.Y;
+    Z = J.Z;
+    zc = p.zc;
+    const int t = tile - (lo ? p.L.tile_end[lo - 1] : 0);
+    bxi = t % J.fx;
+    byi = (t / J.fx) % J.fy;
+    bzi = t / (J.fx * J.fy);
   }
   const int xb = bxi * kFO;  // first owned column
   const int x = xb - 2 + lane;      // this lane's column
@@ -318,7 +392,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(Fused2Args<S,
     }
 
     // ---- second time level: P^{n+2}[k-1] on the owned rows
-    if (k - 1 >= z0) {
+    if (two && k - 1 >= z0) {
       const auto rD = slice_rsrc(gD + (size_t)(k - 1) * XY, sbytes);
       auto level2 = [&](auto table) {
 #pragma unroll
@@ -396,8 +470,30 @@ template <typename S, typename CT, typename C>
 static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a,
                                hipStream_t stream) {
   const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
-  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C>), g.grid, g.block, lds, stream, a, g.X, g.Y,
-                     g.Z, g.zc);
+  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false>), g.grid, g.block, lds, stream,
+                     Fused2Ctx<S, CT, C>{a, g.X, g.Y, g.Z, g.zc});
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
+                               const PoolState *st, const PoolLaunch &L, int zc,
+                               hipStream_t stream) {
+  const Fused2Pool p{jobs, st, L, zc};
+  const size_t lds = sizeof(CoefF) * kStageRows * (size_t)L.ncol_max;
+  const dim3 grid((unsigned)tiles), block(64 * kFWaves);
+  switch (precision) {
+  case PFDTD_F32:
+    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true>), grid, block, lds, stream, p);
+    break;
+  case PFDTD_F16:
+    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true>), grid, block, lds, stream, p);
+    break;
+  case PFDTD_BF16:
+    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true>), grid, block, lds, stream, p);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

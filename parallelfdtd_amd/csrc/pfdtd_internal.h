@@ -74,6 +74,73 @@ struct UpdateArgs {
   int scheme;  // 0 forward (Bilbao) boundaries, 2 centred (Kowalczyk) boundaries
 };
 
+// ---- serving pool (pfdtd_pool.hip; the two-level pool launch lives in pfdtd_fused.hip)
+constexpr int kPoolMaxJobs = 64;
+
+// One resident job. Field buffers F[0..3]: jobs advanced by single steps use F[0] and F[1];
+// jobs advanced by the two-level sweep (fused != 0) own all four.
+struct PoolJobDev {
+  const uint8_t *pos;
+  const uint8_t *mat;
+  const uint16_t *pm;  // pos | mat << 8 (two-level jobs)
+  void *F[4];
+  const CoefF *coef;
+  int ncol;  // material columns of the table the job's domain addresses (its n_mat)
+  CoefF air;
+  int air_uniform;
+  float lam2;
+  int X, Y, Z;
+  int gx, gy;      // single-step tiles along x and y (z tiles follow from the tile count)
+  int fx, fy;      // two-level tiles along x and y
+  int fused;       // advanced by the two-level sweep
+  const int64_t *rec_idx;
+  const int32_t *rec_xyz;
+  double *resp;  // [n_rec][n_steps]
+  const int64_t *src_idx;
+  const int32_t *src_xyz;
+  const int32_t *src_kind;
+  const float *sig;  // [n_src][n_samples]
+  int n_rec, n_src, n_samples, n_steps;
+  int soft_overwrite, scale_exp;
+  int distinct;  // sources on distinct voxels: injection may run in parallel
+};
+
+// Per-job progress in device memory: steps done and the buffers holding P^n (cur) and
+// P^{n-1} (past).
+struct PoolState {
+  int step;
+  int cur, past;
+};
+
+// The two buffers of a four-buffer job that hold neither P^n nor P^{n-1}, in index order.
+__host__ __device__ inline void pool_spare(int cur, int past, int &o1, int &o2) {
+  o1 = -1;
+  o2 = -1;
+  for (int i = 0; i < 4; ++i)
+    if (i != cur && i != past) {
+      if (o1 < 0)
+        o1 = i;
+      else
+        o2 = i;
+    }
+}
+
+// Launch table, passed by value: the jobs of one launch and their cumulative tile counts.
+struct PoolLaunch {
+  int n;
+  int ncol_max;  // largest material column count of the launch's jobs (LDS bytes per block)
+  int slot[kPoolMaxJobs];
+  int tile_end[kPoolMaxJobs];
+};
+
+// Two-level sweep of every job of a launch (two steps, or one for a job with one step left),
+// then per job: gather/inject of the second step and the buffer rotation.
+hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
+                               const PoolState *st, const PoolLaunch &L, int zc,
+                               hipStream_t stream);
+hipError_t launch_pool_inject2(int precision, const PoolJobDev *jobs, PoolState *st,
+                               const PoolLaunch &L, hipStream_t stream);
+
 // Launchers (all asynchronous on `stream`; return the launch status).
 hipError_t launch_update_f32(const UpdateGeom &g, const UpdateArgs<float, CoefF, float> &a,
                              hipStream_t stream);

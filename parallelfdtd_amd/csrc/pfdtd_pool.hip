@@ -2,15 +2,19 @@
 // sizes, FP32 / FP16 / BF16 storage) resident in one HBM arena and advanced together.
 //
 // Device side. Every resident job has a descriptor (PoolJobDev: its buffers, dimensions,
-// coefficients, sources and receivers) and a state word pair (step, parity) in device memory,
-// indexed by the job's slot. One step of all resident jobs of one storage type is ONE launch of
-// pool_update_kernel, whose flattened grid enumerates every job's tiles (the block finds its
-// job by a binary search over the launch table's tile prefix), followed by ONE launch of
-// pool_inject_kernel (one block per job) that gathers the receivers, injects the next samples
-// and then flips the job's parity and advances its step. Because the kernels read the step and
-// the parity from device memory, a bundle of steps is replay-invariant: it is captured once
-// into a hipGraph per membership of the pool and replayed until a job joins or leaves. A job
-// whose last step is done is skipped by the kernels until the host drops it from the launch.
+// coefficients, sources and receivers) and a state (step, current buffer, past buffer) in device
+// memory, indexed by the job's slot. Jobs are advanced in rounds of two steps. A job whose
+// sources avoid the never-updated first and last z slices owns four field buffers and is
+// advanced by the two-level sweep (srl_update2_kernel in pool mode, pfdtd_fused.hip): ONE launch
+// per storage type enumerates the sweep tiles of every such job (a block finds its job by a
+// binary search over the launch table's tile prefix) and does two steps per job, or one for a
+// job with a single step left; pool_inject2_kernel (one block per job) then gathers and injects
+// the second step and rotates the buffer roles. Other jobs take two single steps per round:
+// pool_update_kernel (flattened single-step tiles) and pool_inject_kernel, which gathers,
+// injects the next samples and swaps current and past. Because the kernels read the state from
+// device memory, a bundle of rounds is replay-invariant: it is captured once into a hipGraph per
+// membership of the pool and replayed until a job joins or leaves. A job whose last step is
+// done is skipped by the kernels until the host drops it from the launch.
 //
 // Host side. Jobs are admitted (pfdtd_pool_submit) only when their buffers fit in the arena's
 // free space and a slot is free; the arena is one allocation made at pool creation, so
@@ -32,43 +36,7 @@
 
 namespace pfdtd {
 
-constexpr int kPoolMaxJobs = 64;
 constexpr int kPoolTW = 64;  // tile width: rooms of any width waste fewer than 64 lanes per row
-
-struct PoolJobDev {
-  const uint8_t *pos;
-  const uint8_t *mat;
-  void *A;  // field buffers: parity 0 -> A current, B past; parity 1 -> swapped
-  void *B;
-  const CoefF *coef;
-  int ncol;  // material columns of the table the job's domain addresses (its n_mat)
-  CoefF air;
-  int air_uniform;
-  float lam2;
-  int X, Y, Z;
-  int gx, gy;  // tiles along x and y (z tiles follow from the tile count)
-  const int64_t *rec_idx;
-  double *resp;  // [n_rec][n_steps]
-  const int64_t *src_idx;
-  const int32_t *src_kind;
-  const float *sig;  // [n_src][n_samples]
-  int n_rec, n_src, n_samples, n_steps;
-  int soft_overwrite, scale_exp;
-  int distinct;  // sources on distinct voxels: injection may run in parallel
-};
-
-struct PoolState {
-  int step;    // steps completed
-  int parity;  // which buffer holds the current field
-};
-
-// Launch table, passed by value: the jobs of one launch and their cumulative tile counts.
-struct PoolLaunch {
-  int n;
-  int ncol_max;  // largest material column count of the launch's jobs (LDS bytes per block)
-  int slot[kPoolMaxJobs];
-  int tile_end[kPoolMaxJobs];
-};
 
 template <typename S, int RY>
 __global__ __launch_bounds__(kTX) void pool_update_kernel(const PoolJobDev *__restrict__ jobs,
@@ -92,8 +60,8 @@ __global__ __launch_bounds__(kTX) void pool_update_kernel(const PoolJobDev *__re
   const int r = t / J.gx;
   const int by = r % J.gy;
   const int bz = r / J.gy;
-  const S *P = (const S *)(st.parity ? J.B : J.A);
-  S *Pp = (S *)(st.parity ? J.A : J.B);
+  const S *P = (const S *)J.F[st.cur];
+  S *Pp = (S *)J.F[st.past];
   // every thread of the block serves the same job, so the whole block reaches the barrier
   extern __shared__ __align__(16) unsigned char s_dyn[];
   CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
@@ -102,6 +70,30 @@ __global__ __launch_bounds__(kTX) void pool_update_kernel(const PoolJobDev *__re
                            J.lam2, J.X, J.Y, J.Z, zc, bx, by, bz);
 }
 
+// Gather/inject arguments of job J at its current state.
+inline __device__ InjectArgs pool_inject_args(const PoolJobDev &J, int gather_step,
+                                              int inject_step) {
+  InjectArgs a;
+  a.rec_idx = J.rec_idx;
+  a.n_rec = J.n_rec;
+  a.resp = J.resp;
+  a.resp_steps = J.n_steps;
+  a.gather_step = J.n_rec > 0 ? gather_step : -1;
+  a.src_idx = J.src_idx;
+  a.src_kind = J.src_kind;
+  a.n_src = J.n_src;
+  a.sig_f = J.sig;
+  a.sig_d = nullptr;
+  a.n_samples = J.n_samples;
+  a.inject_step = (J.n_src > 0 && inject_step < J.n_steps) ? inject_step : -1;
+  a.soft_overwrite = J.soft_overwrite;
+  a.scale_exp = J.scale_exp;
+  a.distinct = J.distinct;
+  return a;
+}
+
+// After a single step: gather step n on P^{n+1} (written over the past buffer), inject step
+// n+1 into it, then P^{n+1} becomes the current field.
 template <typename S>
 __global__ __launch_bounds__(256) void pool_inject_kernel(const PoolJobDev *__restrict__ jobs,
                                                           PoolState *__restrict__ state,
@@ -110,26 +102,33 @@ __global__ __launch_bounds__(256) void pool_inject_kernel(const PoolJobDev *__re
   const PoolJobDev &J = jobs[slot];
   const PoolState st = state[slot];
   if (st.step >= J.n_steps) return;
-  InjectArgs a;
-  a.rec_idx = J.rec_idx;
-  a.n_rec = J.n_rec;
-  a.resp = J.resp;
-  a.resp_steps = J.n_steps;
-  a.gather_step = J.n_rec > 0 ? st.step : -1;
-  a.src_idx = J.src_idx;
-  a.src_kind = J.src_kind;
-  a.n_src = J.n_src;
-  a.sig_f = J.sig;
-  a.sig_d = nullptr;
-  a.n_samples = J.n_samples;
-  a.inject_step = (J.n_src > 0 && st.step + 1 < J.n_steps) ? st.step + 1 : -1;
-  a.soft_overwrite = J.soft_overwrite;
-  a.scale_exp = J.scale_exp;
-  a.distinct = J.distinct;
-  // the update just wrote the new field into the past buffer
-  inject_gather_block<S>((S *)(st.parity ? J.A : J.B), a);
+  inject_gather_block<S>((S *)J.F[st.past], pool_inject_args(J, st.step, st.step + 1));
   __syncthreads();
-  if (threadIdx.x == 0) state[slot] = PoolState{st.step + 1, st.parity ^ 1};
+  if (threadIdx.x == 0) state[slot] = PoolState{st.step + 1, st.past, st.cur};
+}
+
+// After a two-level sweep (pool_update2 in pfdtd_fused.hip, which gathered step n and
+// injected step n+1 itself): with two steps done, gather step n+1 on P^{n+2} and inject step
+// n+2 into it; P^{n+2} and P^{n+1} (the spare buffers o2, o1) become current and past. With
+// one step done (the job's last), P^{n+1} in o1 becomes current.
+template <typename S>
+__global__ __launch_bounds__(256) void pool_inject2_kernel(const PoolJobDev *__restrict__ jobs,
+                                                           PoolState *__restrict__ state,
+                                                           PoolLaunch L) {
+  const int slot = L.slot[blockIdx.x];
+  const PoolJobDev &J = jobs[slot];
+  const PoolState st = state[slot];
+  const int remaining = J.n_steps - st.step;
+  if (remaining <= 0) return;
+  int o1, o2;
+  pool_spare(st.cur, st.past, o1, o2);
+  if (remaining >= 2) {
+    inject_gather_block<S>((S *)J.F[o2], pool_inject_args(J, st.step + 1, st.step + 2));
+    __syncthreads();
+    if (threadIdx.x == 0) state[slot] = PoolState{st.step + 2, o2, o1};
+  } else if (threadIdx.x == 0) {
+    state[slot] = PoolState{st.step + 1, o1, st.cur};
+  }
 }
 
 template <typename S>
@@ -161,6 +160,18 @@ hipError_t launch_pool_update(int precision, int ry, int tiles, const PoolJobDev
   case PFDTD_BF16: return launch_pool_update_t<b16>(ry, tiles, jobs, st, L, zc, stream);
   default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_pool_inject2(int precision, const PoolJobDev *jobs, PoolState *st,
+                               const PoolLaunch &L, hipStream_t stream) {
+  const dim3 grid(L.n), block(256);
+  switch (precision) {
+  case PFDTD_F32: hipLaunchKernelGGL(pool_inject2_kernel<float>, grid, block, 0, stream, jobs, st, L); break;
+  case PFDTD_F16: hipLaunchKernelGGL(pool_inject2_kernel<h16>, grid, block, 0, stream, jobs, st, L); break;
+  case PFDTD_BF16: hipLaunchKernelGGL(pool_inject2_kernel<b16>, grid, block, 0, stream, jobs, st, L); break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_pool_inject(int precision, const PoolJobDev *jobs, PoolState *st,
@@ -213,12 +224,17 @@ size_t storage_bytes(int precision) {
   }
 }
 
-// Byte offsets of one job's buffers inside its arena block.
+constexpr int kPoolFusedZc = 32;  // z-chunk of the two-level sweep in the pool
+
+// Byte offsets of one job's buffers inside its arena block (a two-level job adds two field
+// buffers, the packed byte plane and the source/receiver coordinates).
 struct JobLayout {
-  size_t A, B, pos, mat, labels, coef, src_idx, src_kind, sig, rec_idx, resp, total;
+  size_t F[4], pos, mat, pm, labels, coef, src_idx, src_xyz, src_kind, sig, rec_idx, rec_xyz,
+      resp, total;
 };
 
-JobLayout layout_job(int64_t N, size_t es, int n_src, int n_samples, int n_rec, int n_steps) {
+JobLayout layout_job(int64_t N, size_t es, int n_src, int n_samples, int n_rec, int n_steps,
+                     bool fused) {
   JobLayout L{};
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -226,16 +242,18 @@ JobLayout layout_job(int64_t N, size_t es, int n_src, int n_samples, int n_rec, 
     o += align_up(bytes ? bytes : 1);
     return at;
   };
-  L.A = take(es * (size_t)N);
-  L.B = take(es * (size_t)N);
+  for (int b = 0; b < 4; ++b) L.F[b] = (b < 2 || fused) ? take(es * (size_t)N) : 0;
   L.pos = take((size_t)N);
   L.mat = take((size_t)N);
+  L.pm = fused ? take(sizeof(uint16_t) * (size_t)N) : 0;
   L.labels = take((size_t)N);
   L.coef = take(sizeof(CoefF) * kCoefEntries);
   L.src_idx = take(sizeof(int64_t) * (size_t)n_src);
+  L.src_xyz = fused ? take(sizeof(int32_t) * 3 * (size_t)n_src) : 0;
   L.src_kind = take(sizeof(int32_t) * (size_t)n_src);
   L.sig = take(sizeof(float) * (size_t)n_src * (size_t)n_samples);
   L.rec_idx = take(sizeof(int64_t) * (size_t)n_rec);
+  L.rec_xyz = fused ? take(sizeof(int32_t) * 3 * (size_t)n_rec) : 0;
   L.resp = take(sizeof(double) * (size_t)n_rec * (size_t)n_steps);
   L.total = o;
   return L;
@@ -249,7 +267,8 @@ struct PoolJobHost {
   int precision = 0;
   int n_steps = 0, n_rec = 0;
   int64_t voxels = 0;
-  int tiles = 0;
+  int tiles = 0;  // tiles of the job's update kernel (two-level or single-step)
+  bool fused = false;
   int ncol = 1;
   size_t off = 0, bytes = 0;
   int64_t steps_enqueued = 0;
@@ -285,8 +304,8 @@ struct pfdtd_pool {
   bool dirty = true;
   std::vector<int64_t> members;  // job ids in the current launch
   int prec_list[3] = {PFDTD_F32, PFDTD_F16, PFDTD_BF16};
-  PoolLaunch launch[3];
-  int tiles[3] = {0, 0, 0};
+  PoolLaunch launch[3][2];  // [storage type][0 single-step jobs, 1 two-level jobs]
+  int tiles[3][2] = {{0, 0}, {0, 0}, {0, 0}};
   hipGraphExec_t exec = nullptr;
   uint64_t exec_last_seq = 0;
   std::vector<std::pair<uint64_t, hipGraphExec_t>> retired;
@@ -388,35 +407,47 @@ int reap(pfdtd_pool *p, bool block) {
   return PFDTD_OK;
 }
 
+// A bundle is bundle/2 rounds; each round advances two-level jobs by one sweep (two steps, or
+// the last one) and single-step jobs by two steps.
 int enqueue_bundle_kernels(pfdtd_pool *p) {
-  for (int k = 0; k < p->bundle; ++k) {
+  for (int k = 0; k < p->bundle; k += 2) {
     for (int q = 0; q < 3; ++q)
-      if (p->launch[q].n)
-        POOL_TRY(launch_pool_update(p->prec_list[q], p->ry, p->tiles[q], p->d_jobs, p->d_state,
-                                    p->launch[q], p->zc, p->run));
-    for (int q = 0; q < 3; ++q)
-      if (p->launch[q].n)
-        POOL_TRY(launch_pool_inject(p->prec_list[q], p->d_jobs, p->d_state, p->launch[q], p->run));
+      if (p->launch[q][1].n) {
+        POOL_TRY(launch_pool_update2(p->prec_list[q], p->tiles[q][1], p->d_jobs, p->d_state,
+                                     p->launch[q][1], kPoolFusedZc, p->run));
+        POOL_TRY(launch_pool_inject2(p->prec_list[q], p->d_jobs, p->d_state, p->launch[q][1],
+                                     p->run));
+      }
+    for (int half = 0; half < 2; ++half)
+      for (int q = 0; q < 3; ++q)
+        if (p->launch[q][0].n) {
+          POOL_TRY(launch_pool_update(p->prec_list[q], p->ry, p->tiles[q][0], p->d_jobs,
+                                      p->d_state, p->launch[q][0], p->zc, p->run));
+          POOL_TRY(launch_pool_inject(p->prec_list[q], p->d_jobs, p->d_state, p->launch[q][0],
+                                      p->run));
+        }
   }
   return PFDTD_OK;
 }
 
 int rebuild_membership(pfdtd_pool *p) {
   p->members.clear();
-  for (int q = 0; q < 3; ++q) {
-    p->launch[q].n = 0;
-    p->launch[q].ncol_max = 1;
-    p->tiles[q] = 0;
-  }
+  for (int q = 0; q < 3; ++q)
+    for (int f = 0; f < 2; ++f) {
+      p->launch[q][f].n = 0;
+      p->launch[q][f].ncol_max = 1;
+      p->tiles[q][f] = 0;
+    }
   for (auto &kv : p->jobs) {
     PoolJobHost &j = kv.second;
     if (j.phase != kActive) continue;
     const int q = j.precision == PFDTD_F32 ? 0 : (j.precision == PFDTD_F16 ? 1 : 2);
-    PoolLaunch &L = p->launch[q];
+    const int f = j.fused ? 1 : 0;
+    PoolLaunch &L = p->launch[q][f];
     L.slot[L.n] = j.slot;
     L.ncol_max = std::max(L.ncol_max, j.ncol);
-    p->tiles[q] += j.tiles;
-    L.tile_end[L.n] = p->tiles[q];
+    p->tiles[q][f] += j.tiles;
+    L.tile_end[L.n] = p->tiles[q][f];
     ++L.n;
     p->members.push_back(j.id);
   }
@@ -484,6 +515,20 @@ int check_job(const pfdtd_job *j) {
   for (int64_t i = 0; i < N; ++i) mx = lab[i] > mx ? lab[i] : mx;
   if (mx > j->n_mat) return pfail(PFDTD_E_DOMAIN, "label %u names material %u of %d", mx, mx - 1u, j->n_mat);
   return PFDTD_OK;
+}
+
+// Whether a job is advanced by the two-level sweep: not when a source sits on the first or last
+// z slice (never updated, but read by its neighbours: the single-step order is needed there)
+// or when one z slice exceeds the sweep's 32-bit buffer range.
+bool job_fused(const pfdtd_job *j) {
+  const size_t es = storage_bytes(j->precision);
+  if ((uint64_t)j->X * (uint64_t)j->Y * es >= (1ull << 32)) return false;
+  for (int s = 0; s < j->n_src; ++s) {
+    const int32_t *q = &j->src_xyz[3 * s];
+    const bool in = q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < j->X && q[1] < j->Y && q[2] < j->Z;
+    if (in && (q[2] == 0 || q[2] == j->Z - 1)) return false;
+  }
+  return true;
 }
 
 int64_t grid_index(const int32_t *q, int X, int Y, int Z) {
@@ -554,7 +599,7 @@ int pfdtd_pool_set_options(pfdtd_pool *p, int bundle_steps, int use_graph, int r
   // tile geometry of resident jobs depends on rows / zchunk: change only on an empty pool
   if ((rows != p->ry || zchunk != p->zc) && !p->jobs.empty())
     return pfail(PFDTD_E_STATE, "tiling can change only while the pool is empty");
-  p->bundle = bundle_steps;
+  p->bundle = bundle_steps + (bundle_steps & 1);  // whole rounds of two steps
   p->use_graph = use_graph ? 1 : 0;
   p->ry = rows;
   p->zc = zchunk;
@@ -569,7 +614,9 @@ int pfdtd_pool_job_bytes(const pfdtd_job *j, int64_t *bytes) {
   if (!es || j->X < 1 || j->Y < 1 || j->Z < 1 || j->n_steps < 0 || j->n_src < 0 || j->n_rec < 0)
     return pfail(PFDTD_E_ARG, "bad job dimensions or precision");
   const int64_t N = (int64_t)j->X * j->Y * j->Z;
-  *bytes = (int64_t)layout_job(N, es, j->n_src, j->n_samples, j->n_rec, j->n_steps).total;
+  *bytes = (int64_t)layout_job(N, es, j->n_src, j->n_samples, j->n_rec, j->n_steps,
+                              j->n_src == 0 || j->src_xyz ? job_fused(j) : false)
+               .total;
   return PFDTD_OK;
 }
 
@@ -601,7 +648,8 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   const int X = j->X, Y = j->Y, Z = j->Z;
   const int64_t N = (int64_t)X * Y * Z;
   const size_t es = storage_bytes(j->precision);
-  const JobLayout L = layout_job(N, es, j->n_src, j->n_samples, j->n_rec, j->n_steps);
+  const bool fused = job_fused(j);
+  const JobLayout L = layout_job(N, es, j->n_src, j->n_samples, j->n_rec, j->n_steps, fused);
   size_t off = 0;
   if (!arena_alloc(p, L.total, &off)) {
     size_t big = 0;
@@ -628,8 +676,8 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   PoolJobDev d{};
   d.pos = (const uint8_t *)(base + L.pos);
   d.mat = (const uint8_t *)(base + L.mat);
-  d.A = base + L.A;
-  d.B = base + L.B;
+  d.pm = fused ? (const uint16_t *)(base + L.pm) : nullptr;
+  for (int b = 0; b < 4; ++b) d.F[b] = (b < 2 || fused) ? base + L.F[b] : nullptr;
   d.coef = (const CoefF *)(base + L.coef);
   d.ncol = j->n_mat;  // labels were checked to name materials 0..n_mat-1 only
   d.air = cf[6 * 256];
@@ -642,14 +690,21 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   d.gx = (X + kPoolTW - 1) / kPoolTW;
   d.gy = (Y + rows_per_tile - 1) / rows_per_tile;
   const int gz = (Z - 2 + p->zc - 1) / p->zc;
-  const int64_t tiles = (int64_t)d.gx * d.gy * gz;
+  const FusedGeom fg = plan_fused(X, Y, Z, kPoolFusedZc, j->precision == PFDTD_F32 ? 0 : 2);
+  d.fx = (int)fg.grid.x;
+  d.fy = (int)fg.grid.y;
+  d.fused = fused ? 1 : 0;
+  const int64_t tiles = fused ? (int64_t)fg.grid.x * fg.grid.y * fg.grid.z
+                              : (int64_t)d.gx * d.gy * gz;
   if (tiles > (int64_t)1 << 30) {
     arena_free(p, off, L.total);
     return pfail(PFDTD_E_ARG, "job grid too large for one launch");
   }
   d.rec_idx = (const int64_t *)(base + L.rec_idx);
+  d.rec_xyz = fused ? (const int32_t *)(base + L.rec_xyz) : nullptr;
   d.resp = (double *)(base + L.resp);
   d.src_idx = (const int64_t *)(base + L.src_idx);
+  d.src_xyz = fused ? (const int32_t *)(base + L.src_xyz) : nullptr;
   d.src_kind = (const int32_t *)(base + L.src_kind);
   d.sig = (const float *)(base + L.sig);
   d.n_rec = j->n_rec;
@@ -665,7 +720,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
     std::sort(v.begin(), v.end());
     d.distinct = std::adjacent_find(v.begin(), v.end()) == v.end() ? 1 : 0;
   }
-  const PoolState st0{0, 0};
+  const PoolState st0{0, 0, 1};
 
   // set-up on the upload stream, concurrent with the resident jobs' steps
   hipStream_t u = p->upload;
@@ -673,8 +728,14 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   if (e == hipSuccess)
     e = launch_classify_labels((const uint8_t *)(base + L.labels), (uint8_t *)(base + L.pos),
                                (uint8_t *)(base + L.mat), X, Y, Z, u);
-  if (e == hipSuccess) e = hipMemsetAsync(base + L.A, 0, es * (size_t)N, u);
-  if (e == hipSuccess) e = hipMemsetAsync(base + L.B, 0, es * (size_t)N, u);
+  for (int b = 0; b < 4 && e == hipSuccess; ++b)
+    if (d.F[b]) e = hipMemsetAsync(d.F[b], 0, es * (size_t)N, u);
+  if (e == hipSuccess && fused)
+    e = launch_pack_pm(d.pos, d.mat, (uint16_t *)(base + L.pm), N, u);
+  if (e == hipSuccess && fused && j->n_src)
+    e = hipMemcpyAsync(base + L.src_xyz, j->src_xyz, sizeof(int32_t) * 3 * j->n_src, hipMemcpyHostToDevice, u);
+  if (e == hipSuccess && fused && j->n_rec)
+    e = hipMemcpyAsync(base + L.rec_xyz, j->rec_xyz, sizeof(int32_t) * 3 * j->n_rec, hipMemcpyHostToDevice, u);
   if (e == hipSuccess && j->n_rec)
     e = hipMemsetAsync(base + L.resp, 0, sizeof(double) * (size_t)j->n_rec * j->n_steps, u);
   if (e == hipSuccess) e = hipMemcpyAsync(base + L.coef, cf.data(), sizeof(CoefF) * kCoefEntries, hipMemcpyHostToDevice, u);
@@ -706,7 +767,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
     a.soft_overwrite = d.soft_overwrite;
     a.scale_exp = j->scale_exp;
     a.distinct = d.distinct;
-    e = launch_inject_gather(j->precision, d.A, a, u);
+    e = launch_inject_gather(j->precision, d.F[0], a, u);
   }
   if (e == hipSuccess) e = hipMemcpyAsync(p->d_jobs + slot, &d, sizeof(d), hipMemcpyHostToDevice, u);
   if (e == hipSuccess) e = hipMemcpyAsync(p->d_state + slot, &st0, sizeof(st0), hipMemcpyHostToDevice, u);
@@ -722,6 +783,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   h.id = p->next_id++;
   h.slot = slot;
   h.precision = j->precision;
+  h.fused = fused;
   h.ncol = j->n_mat;
   h.n_steps = j->n_steps;
   h.n_rec = j->n_rec;

@@ -131,3 +131,25 @@ def test_serve_stream_matches_alone(gpu):
     assert rep["latency_s"]["p50"] > 0 and rep["latency_s"]["p99"] >= rep["latency_s"]["p50"]
     for j in (0, 4, 9):
         assert np.array_equal(rep["results"][j], S.run_alone(S.make_job(j, 8, 40))), f"job {j}"
+
+
+@pytest.mark.parametrize("bundle", [1, 5, 16])
+def test_pool_two_level_and_single_step_jobs_mixed(gpu, bundle):
+    """Jobs advanced by the two-level sweep (four buffers, two steps per launch, one for the
+    last odd step) and jobs advanced by single steps (a source on the never-updated first z
+    slice) in one pool, with step counts 1, 2, 3 and odd lengths: each equals the job alone."""
+    base = small_jobs(5, first=20)
+    specs = []
+    for k, n_steps in enumerate([1, 2, 3, 37, 64]):
+        j = base[k]
+        specs.append(S.JobSpec(**{**j.__dict__, "n_steps": n_steps}))
+    single = S.JobSpec(**{**base[0].__dict__, "n_steps": 45})
+    src = single.src_xyz.copy()
+    src[0, 2] = 0  # on the first z slice: this job takes single steps
+    single.src_xyz = src
+    specs.append(single)
+    alone = [S.run_alone(j) for j in specs]
+    with S.Pool(0, arena_bytes=1 << 30, bundle=bundle) as pool:
+        got = drive(pool, specs, schedule=[0, 0, 1, 1, 2, 0])
+    for k, j in enumerate(specs):
+        assert np.array_equal(got[k], alone[k]), f"job {k} ({j.n_steps} steps, prec {j.precision})"
