@@ -1,8 +1,9 @@
 // pfdtd_fused.hip -- two SRL_FORWARD time steps per sweep (temporal blocking).
 //
-// Reads P^n (A), P^{n-1} (B) and the byte planes once and writes P^{n+1} (C) and P^{n+2} (D):
-// 18 bytes of HBM traffic per two FP32 node-updates instead of 28. The four field buffers are
-// distinct, so blocks never read what another block writes and need no synchronisation.
+// Reads P^n (A), P^{n-1} (B) and the packed position|material plane once and writes P^{n+1} (C)
+// and P^{n+2} (D): 18 bytes of HBM traffic per two FP32 node-updates instead of 28. The four
+// field buffers are distinct, so blocks never read what another block writes and need no
+// synchronisation between blocks.
 //
 // Tiling: each wavefront owns a 60-column x TY-row x-y tile (TY = 6 for FP32, 4 for FP64, 10 for
 // 16-bit storage) and one z-chunk. Its 64 lanes cover the 60 owned columns plus two halo
@@ -12,7 +13,13 @@
 // the wave marches z with P^n of three slices and P^{n+1} of three slices in registers, the
 // second time level trailing the first by one slice. Every field and byte access of a slice is a
 // buffer instruction (descriptor of the slice in scalar registers, loop-invariant 32-bit lane
-// offsets), and all loads of a slice are issued before any is consumed.
+// offsets), and all loads of a slice are issued before any is consumed. The four waves of a
+// block step z together (one barrier per slice) so that their overlapping halo rows meet in
+// cache, and the boundary coefficients are gathered from the block's LDS copy of the table.
+//
+// The same kernel serves two callers: a context's domain (POOL = false) and the two-level jobs
+// of one serving-pool launch (POOL = true: the block finds its job in the launch table and takes
+// its arguments from the job descriptor; a job with one step left runs only the first level).
 //
 // Source injection of step n+1 and the receiver gather of step n happen inside the kernel on
 // P^{n+1}, in the reference's order (gather before the next injection; sources in index order),
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
   const size_t XY = (size_t)X * (size_t)Y;
   const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^32
   const int xc = min(max(x, 0), X - 1);
-  // Rows r = 0..11 of the P^n block are y = yt - 2 + r (clamped to the grid for loads).
+  // Rows r = 0..NA-1 of the P^n block are y = yt - 2 + r (clamped to the grid for loads).
   uint32_t ro[NA];
   uint32_t imask = 0;  // bit r: node (x, y_r) is updated (not on the outer layer)
 #pragma unroll
@@ -278,7 +285,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
     return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
   };
   const bool air_uniform = a.air_uniform != 0;
-  // wave-uniform: does any of the ten packed rows hold a voxel other than air (0x86)?
+  // wave-uniform: does any of the NQ packed rows hold a voxel other than air (0x86)?
   auto any_nonair = [&](const uint32_t (&b)[NB]) -> bool {
     uint32_t m = 0;
 #pragma unroll
@@ -291,9 +298,9 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
   S *__restrict__ gD = a.Dn;
   const uint16_t *__restrict__ gpm = a.pm;
 
-  // One z-iteration: first level at slice k (rows 1..10), second level at slice k-1 (rows 2..9).
-  // am/ac/ap: P^n at k-1, k, k+1 (12 rows); q0/q1/q2: P^{n+1} at k-2, k-1, k (rows 1..10 at
-  // index row-1); bp/bc: packed bytes at k-1 and k (rows 1..10).
+  // One z-iteration: first level at slice k (rows 1..NQ), second level at slice k-1 (rows
+  // 2..NQ-1). am/ac/ap: P^n at k-1, k, k+1 (NA rows); q0/q1/q2: P^{n+1} at k-2, k-1, k (rows
+  // 1..NQ at index row-1); bp/bc: packed bytes at k-1 and k (rows 1..NQ).
   auto body = [&](C (&am)[NA], C (&ac)[NA], C (&ap)[NA], C (&q0)[NQ], C (&q1)[NQ], C (&q2)[NQ],
                   uint32_t (&bp)[NB], uint32_t (&bc)[NB], int k) {
     // the block's four waves load overlapping halo rows: keeping them on the same slice lets
