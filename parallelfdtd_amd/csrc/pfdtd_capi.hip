@@ -69,6 +69,7 @@ struct pfdtd_ctx {
   int64_t X = 0, Y = 0, Z = 0, N = 0;
   uint8_t *d_pos = nullptr, *d_mat = nullptr;
   uint16_t *d_pm = nullptr;  // pos | mat << 8, built for the two-level update (pm_ok: current)
+  uint32_t *d_air_rows = nullptr;  // its air-row bits (built with it; null: not available)
   bool pm_ok = false;
   int64_t dom_cap = 0;  // voxels the position/material buffers hold
   bool has_domain = false;
@@ -320,6 +321,7 @@ int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
     c->dom_cap = 0;
   }
   c->pm_ok = false;
+  dfree(c->d_air_rows);  // sized by the grid's shape, not only its voxel count
   // fields sized for the previous grid are dropped
   free_fields(c);
   if (!c->d_pos) {
@@ -521,6 +523,7 @@ template <typename S, typename CT, typename C>
 Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step) {
   Fused2Args<S, CT, C> a;
   a.pm = c->d_pm;
+  a.air_rows = c->d_air_rows;
   a.A = (const S *)c->d_P;
   a.B = (const S *)c->d_Pp;
   a.Cn = (S *)c->d_Pp2;
@@ -631,6 +634,7 @@ int pfdtd_ctx_destroy(pfdtd_ctx *c) {
   dfree(c->d_pos);
   dfree(c->d_mat);
   dfree(c->d_pm);
+  dfree(c->d_air_rows);
   dfree(c->d_scan);
   dfree(c->d_coef_f);
   dfree(c->d_coef_d);
@@ -1043,6 +1047,17 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
       fused = false;
     } else {
       HIP_TRY(launch_pack_pm(c->d_pos, c->d_mat, c->d_pm, c->N, c->stream));
+      // the air-row bits only save loads: without room for them the sweep reads every row
+      const size_t words = air_rows_words((int)c->X, (int)c->Y, (int)c->Z);
+      if (!c->d_air_rows && hipMalloc((void **)&c->d_air_rows, sizeof(uint32_t) * words) != hipSuccess) {
+        (void)hipGetLastError();
+        c->d_air_rows = nullptr;
+      }
+      if (c->d_air_rows &&
+          launch_air_rows(c->d_pm, c->d_air_rows, (int)c->X, (int)c->Y, (int)c->Z, c->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        dfree(c->d_air_rows);
+      }
       c->pm_ok = true;
     }
   }

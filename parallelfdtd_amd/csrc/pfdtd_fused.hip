@@ -5,7 +5,7 @@
 // field buffers are distinct, so blocks never read what another block writes and need no
 // synchronisation between blocks.
 //
-// Tiling: each wavefront owns a 60-column x TY-row x-y tile (TY = 6 for FP32, 4 for FP64, 10 for
+// Tiling: each wavefront owns a 60-column x TY-row x-y tile (TY = 6 for FP32, 4 for FP64, 8 for
 // 16-bit storage) and one z-chunk. Its 64 lanes cover the 60 owned columns plus two halo
 // columns on each side. P^{n+1} is computed on the owned tile plus a one-voxel ring (lanes
 // 1..62, TY+2 rows), from P^n on a two-voxel ring (TY+4 rows); P^{n+2} is then computed on the
@@ -16,6 +16,10 @@
 // offsets), and all loads of a slice are issued before any is consumed. The four waves of a
 // block step z together (one barrier per slice) so that their overlapping halo rows meet in
 // cache, and the boundary coefficients are gathered from the block's LDS copy of the table.
+// Rooms are mostly air: an air-row plane (one bit per slice, x tile and row, 1/480 of the
+// packed plane's bytes) marks the tile rows whose voxels are all plain air, and a wave whose
+// P^{n+1} rows of a slice are all marked takes the air value instead of loading the packed
+// plane -- the value it would have loaded, so the results do not change.
 //
 // The same kernel serves two callers: a context's domain (POOL = false) and the two-level jobs
 // of one serving-pool launch (POOL = true: the block finds its job in the launch table and takes
@@ -27,6 +31,7 @@
 #include "../../include/pfdtd.h"
 #include "pfdtd_device.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace pfdtd {
@@ -115,12 +120,13 @@ __device__ __forceinline__ unsigned pm_of(const uint32_t (&b)[NB], int j) {
 }
 
 // Owned rows per wave tile (must be even: the byte planes of two rows share a register).
-// Measured on the 2048x1536x768 room: 2-byte fields 10 (166 VGPRs, three waves per SIMD; 8 %
-// faster than 8 rows). FP32 6, the most that keeps the kernel within 128 VGPRs (four waves per
-// SIMD; 3 % faster than 4 rows). FP64: 4.
+// Measured on the 2048x1536x768 room: 2-byte fields 8 (120 VGPRs, four waves per SIMD; 11 %
+// faster than 10 rows at three waves, 10 % faster than 6 rows at five). FP32 6, the most that
+// keeps the kernel within 128 VGPRs (four waves per SIMD; 3 % faster than 4 rows at five).
+// FP64: 4.
 template <typename S>
 struct FusedTile {
-  static constexpr int TY = 10;
+  static constexpr int TY = 8;
 };
 template <>
 struct FusedTile<float> {
@@ -156,7 +162,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
   __shared__ int s_src[kFWaves][kFListCap];
   __shared__ int s_rec[kFWaves][kFListCap];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   // XCD-aware tile order: consecutive block ids are dealt round-robin to the 8 XCDs (each with
   // its own L2); remapping them so that every XCD walks a contiguous run of x-fastest tiles keeps
   // the halo columns and rows that neighbouring tiles share in one L2 while both are resident.
@@ -197,6 +203,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
     int o1, o2;
     pool_spare(st.cur, st.past, o1, o2);
     a.pm = J.pm;
+    a.air_rows = J.air_rows;
     a.A = (const S *)J.F[st.cur];
     a.B = (const S *)J.F[st.past];
     a.Cn = (S *)J.F[o1];
@@ -297,6 +304,24 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
   S *__restrict__ gC = a.Cn;
   S *__restrict__ gD = a.Dn;
   const uint16_t *__restrict__ gpm = a.pm;
+  // This wave's P^{n+1} rows yt-1 .. yt+kFTY are bits yt .. yt+NQ-1 of its tile's air-row
+  // words (bit b of word w: row 32 w + b - 1). Waves past the grid count as air: every one of
+  // their rows is masked.
+  // The words are read through the constant address space (scalar loads, counted apart from
+  // the slice's vector loads), one slice ahead of their use.
+  typedef const __attribute__((address_space(4))) uint32_t *ConstU32;
+  const uint32_t *__restrict__ gar = a.air_rows;
+  const int nyw = Y / 32 + 2;
+  const size_t ar_slice = (size_t)((X + kFO - 1) / kFO) * (size_t)nyw;
+  const uint32_t *ar_tile = gar ? gar + (size_t)bxi * nyw + (yt >> 5) : nullptr;
+  auto air_window = [&](int k) -> uint64_t {
+    if (!gar || k < 1 || k > Z - 2) return 0;
+    if (yt >= Y) return ~0ull;
+    const ConstU32 w = (ConstU32)(ar_tile + (size_t)k * ar_slice);
+    return (((uint64_t)w[1] << 32) | w[0]) >> (yt & 31);
+  };
+  constexpr uint64_t kAllRows = (1ull << NQ) - 1ull;
+  uint64_t ar_win = air_window(z0 - 1);  // the window of the slice the next body call updates
 
   // One z-iteration: first level at slice k (rows 1..NQ), second level at slice k-1 (rows
   // 2..NQ-1). am/ac/ap: P^n at k-1, k, k+1 (NA rows); q0/q1/q2: P^{n+1} at k-2, k-1, k (rows
@@ -316,18 +341,26 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
       for (int r = 0; r < NA; ++r) ap[r] = (C)0;
     }
     C past[NQ];
+    const bool air1 = upd1 && (ar_win & kAllRows) == kAllRows;  // all P^{n+1} rows plain air
+    ar_win = air_window(k + 1);
     if (upd1) {
       const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
-      const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
 #pragma unroll
       for (int j = 0; j < NQ; ++j) past[j] = Fmt<S>::val(bld<S>(rB, ro[j + 1]));
-      uint32_t pmv[NQ];
+      if (air1) {
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) pmv[j] = bld<uint16_t>(rPM, ro[j + 1]);
-      // every load of the slice is in flight before any result is combined
-      __builtin_amdgcn_sched_barrier(0);
+        for (int i = 0; i < NB; ++i) bc[i] = 0x00860086u;
+      } else {
+        const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
+        uint32_t pmv[NQ];
 #pragma unroll
-      for (int i = 0; i < NB; ++i) bc[i] = pmv[2 * i] | (pmv[2 * i + 1] << 16);
+        for (int j = 0; j < NQ; ++j) pmv[j] = bld<uint16_t>(rPM, ro[j + 1]);
+        // every load of the slice is in flight before any result is combined
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) bc[i] = pmv[2 * i] | (pmv[2 * i + 1] << 16);
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) past[j] = (C)0;
@@ -349,7 +382,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
         q2[j] = (upd1 && ((imask >> r) & 1u)) ? Fmt<S>::round(v) : (C)0;
       }
     };
-    if (any_nonair(bc))
+    if (air1 ? !air_uniform : any_nonair(bc))
       level1(std::true_type());
     else
       level1(std::false_type());
@@ -456,6 +489,44 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
     body(A2, A0, A1, Q2, Q0, Q1, B2, B0, k);
     if (++k > k1) break;
   }
+}
+
+// Air-row plane: bit b of word (z * ntx + t) * nyw + w is set when every voxel of row
+// y = 32 w + b - 1 of slice z in columns [60 t - 1, 60 t + 61) -- the columns whose first-level
+// results wave tile t uses -- is plain air (packed value 0x0086: position 0x86, material 0);
+// rows outside the grid are set. One wave per word, lanes over the columns; blocks stride over
+// words and slices.
+__global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict__ pm,
+                                                      uint32_t *__restrict__ out, int X, int Y,
+                                                      int Z, int ntx, int nyw) {
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int x = t * kFO - 1 + lane;
+  const bool col = lane < kFO + 2 && x >= 0 && x < X;
+  for (int z = blockIdx.z; z < Z; z += gridDim.z) {
+    const uint16_t *slice = pm + (size_t)z * (size_t)X * (size_t)Y;
+    for (int w = blockIdx.y; w < nyw; w += gridDim.y) {
+      uint32_t word = 0;
+      for (int b = 0; b < 32; ++b) {
+        const int y = 32 * w + b - 1;
+        const bool other = col && y >= 0 && y < Y && slice[(size_t)y * X + x] != 0x0086u;
+        if (__ballot(other) == 0) word |= 1u << b;
+      }
+      if (lane == 0) out[((size_t)z * ntx + t) * nyw + w] = word;
+    }
+  }
+}
+
+size_t air_rows_words(int X, int Y, int Z) {
+  return (size_t)((X + kFO - 1) / kFO) * (size_t)(Y / 32 + 2) * (size_t)Z;
+}
+
+hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z,
+                           hipStream_t stream) {
+  const int ntx = (X + kFO - 1) / kFO, nyw = Y / 32 + 2;
+  const dim3 grid((unsigned)ntx, (unsigned)std::min(nyw, 4096), (unsigned)std::min(Z, 4096));
+  hipLaunchKernelGGL(air_rows_kernel, grid, dim3(64), 0, stream, pm, rows, X, Y, Z, ntx, nyw);
+  return hipGetLastError();
 }
 
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision) {

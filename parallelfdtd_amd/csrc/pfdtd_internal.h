@@ -83,6 +83,7 @@ struct PoolJobDev {
   const uint8_t *pos;
   const uint8_t *mat;
   const uint16_t *pm;  // pos | mat << 8 (two-level jobs)
+  const uint32_t *air_rows;  // air-row bits of the two-level wave tiles (two-level jobs)
   void *F[4];
   const CoefF *coef;
   int ncol;  // material columns of the table the job's domain addresses (its n_mat)
@@ -158,6 +159,7 @@ hipError_t launch_update_bf16(const UpdateGeom &g, const UpdateArgs<b16, CoefF, 
 template <typename S, typename CT, typename C>
 struct Fused2Args {
   const uint16_t *pm;  // position byte | material byte << 8 per voxel
+  const uint32_t *air_rows;  // air-row bits of the wave tiles (launch_air_rows), or null
   const S *A;
   const S *B;
   S *Cn;
@@ -188,6 +190,12 @@ struct FusedGeom {
   dim3 grid, block;
 };
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision);
+// One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
+// so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
+// and the kernel that builds them from the packed plane.
+size_t air_rows_words(int X, int Y, int Z);
+hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z,
+                           hipStream_t stream);
 hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
                               hipStream_t stream);
 hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,

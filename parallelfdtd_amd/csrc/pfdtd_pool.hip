@@ -227,14 +227,15 @@ size_t storage_bytes(int precision) {
 constexpr int kPoolFusedZc = 32;  // z-chunk of the two-level sweep in the pool
 
 // Byte offsets of one job's buffers inside its arena block (a two-level job adds two field
-// buffers, the packed byte plane and the source/receiver coordinates).
+// buffers, the packed byte plane, its air-row bits and the source/receiver coordinates).
 struct JobLayout {
-  size_t F[4], pos, mat, pm, labels, coef, src_idx, src_xyz, src_kind, sig, rec_idx, rec_xyz,
-      resp, total;
+  size_t F[4], pos, mat, pm, air_rows, labels, coef, src_idx, src_xyz, src_kind, sig, rec_idx,
+      rec_xyz, resp, total;
 };
 
-JobLayout layout_job(int64_t N, size_t es, int n_src, int n_samples, int n_rec, int n_steps,
-                     bool fused) {
+JobLayout layout_job(int X, int Y, int Z, size_t es, int n_src, int n_samples, int n_rec,
+                     int n_steps, bool fused) {
+  const int64_t N = (int64_t)X * Y * Z;
   JobLayout L{};
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -246,6 +247,7 @@ JobLayout layout_job(int64_t N, size_t es, int n_src, int n_samples, int n_rec, 
   L.pos = take((size_t)N);
   L.mat = take((size_t)N);
   L.pm = fused ? take(sizeof(uint16_t) * (size_t)N) : 0;
+  L.air_rows = fused ? take(sizeof(uint32_t) * air_rows_words(X, Y, Z)) : 0;
   L.labels = take((size_t)N);
   L.coef = take(sizeof(CoefF) * kCoefEntries);
   L.src_idx = take(sizeof(int64_t) * (size_t)n_src);
@@ -613,8 +615,7 @@ int pfdtd_pool_job_bytes(const pfdtd_job *j, int64_t *bytes) {
   const size_t es = storage_bytes(j->precision);
   if (!es || j->X < 1 || j->Y < 1 || j->Z < 1 || j->n_steps < 0 || j->n_src < 0 || j->n_rec < 0)
     return pfail(PFDTD_E_ARG, "bad job dimensions or precision");
-  const int64_t N = (int64_t)j->X * j->Y * j->Z;
-  *bytes = (int64_t)layout_job(N, es, j->n_src, j->n_samples, j->n_rec, j->n_steps,
+  *bytes = (int64_t)layout_job(j->X, j->Y, j->Z, es, j->n_src, j->n_samples, j->n_rec, j->n_steps,
                               j->n_src == 0 || j->src_xyz ? job_fused(j) : false)
                .total;
   return PFDTD_OK;
@@ -649,7 +650,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   const int64_t N = (int64_t)X * Y * Z;
   const size_t es = storage_bytes(j->precision);
   const bool fused = job_fused(j);
-  const JobLayout L = layout_job(N, es, j->n_src, j->n_samples, j->n_rec, j->n_steps, fused);
+  const JobLayout L = layout_job(X, Y, Z, es, j->n_src, j->n_samples, j->n_rec, j->n_steps, fused);
   size_t off = 0;
   if (!arena_alloc(p, L.total, &off)) {
     size_t big = 0;
@@ -677,6 +678,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   d.pos = (const uint8_t *)(base + L.pos);
   d.mat = (const uint8_t *)(base + L.mat);
   d.pm = fused ? (const uint16_t *)(base + L.pm) : nullptr;
+  d.air_rows = fused ? (const uint32_t *)(base + L.air_rows) : nullptr;
   for (int b = 0; b < 4; ++b) d.F[b] = (b < 2 || fused) ? base + L.F[b] : nullptr;
   d.coef = (const CoefF *)(base + L.coef);
   d.ncol = j->n_mat;  // labels were checked to name materials 0..n_mat-1 only
@@ -732,6 +734,8 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
     if (d.F[b]) e = hipMemsetAsync(d.F[b], 0, es * (size_t)N, u);
   if (e == hipSuccess && fused)
     e = launch_pack_pm(d.pos, d.mat, (uint16_t *)(base + L.pm), N, u);
+  if (e == hipSuccess && fused)
+    e = launch_air_rows(d.pm, (uint32_t *)(base + L.air_rows), X, Y, Z, u);
   if (e == hipSuccess && fused && j->n_src)
     e = hipMemcpyAsync(base + L.src_xyz, j->src_xyz, sizeof(int32_t) * 3 * j->n_src, hipMemcpyHostToDevice, u);
   if (e == hipSuccess && fused && j->n_rec)

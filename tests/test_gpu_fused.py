@@ -133,3 +133,61 @@ def test_two_level_sources_on_tile_edges(gpu, prec):
     sc.signals = rng.uniform(-1, 1, (24, sc.n_steps))
     sc.rec_xyz = pts[::-1].copy()
     assert_same(*run_both(sc, prec, zchunk=7))
+
+
+def border_voxel_scene(seed, dims=(185, 80, 26), n_steps=40):
+    """Single solid voxels two and one columns/rows either side of every wave-tile border (x
+    tiles of 60 columns; y tiles of 4, 6 and 8 rows): each puts boundary voxels on exactly one
+    side of the border, so a tile whose air-row bits left out its halo column or row would
+    update a boundary voxel as air."""
+    sc = rooms.scene_random_small(seed, dims=dims, n_mat=3, n_boxes=0, n_src=1, n_rec=1,
+                                  n_steps=n_steps)
+    X, Y, Z = dims
+    xs = sorted({60 * t + d for t in range(1, (X + 59) // 60) for d in (-2, -1, 0, 1)} & set(range(2, X - 2)))
+    ys = sorted({m + d for ty in (4, 6, 8) for m in range(ty, Y, ty) for d in (-2, -1, 0, 1)}
+                & set(range(2, Y - 2)))
+    k = 0
+    for x in xs:
+        for y in ys:
+            z = 2 + (k * 7) % (Z - 4)
+            sc.labels[z, y, x] = 2 + k % 2
+            k += 1
+    rng = np.random.default_rng(seed)
+    n_src = 48
+    sc.src_xyz = np.stack([rng.integers(2, X - 2, n_src), rng.integers(2, Y - 2, n_src),
+                           rng.integers(2, Z - 2, n_src)], axis=1).astype(np.int32)
+    sc.src_kind = np.ones(n_src, np.int32)
+    sc.signals = rng.uniform(-1, 1, (n_src, n_steps))
+    sc.rec_xyz = sc.src_xyz[::3].copy()
+    return sc
+
+
+@pytest.mark.parametrize("prec", [N.F32, N.F64, N.F16, N.BF16])
+def test_two_level_boundary_voxels_on_tile_borders(gpu, prec):
+    sc = border_voxel_scene(87)
+    got, ref, F, F_ref = run_both(sc, prec, zchunk=9)
+    assert np.abs(F_ref[0]).max() > 0
+    assert_same(got, ref, F, F_ref)
+
+
+def test_two_level_domain_reset_with_same_voxel_count(gpu):
+    """One context, three domains: an empty room, the same shape with border voxels, and the
+    transposed shape (same voxel count): each run equals the oracle, so the packed plane and
+    its air-row bits follow the domain."""
+    a = rooms.scene_random_small(88, dims=(122, 60, 20), n_mat=3, n_boxes=0, n_src=4, n_rec=4,
+                                 n_steps=24)
+    b = border_voxel_scene(89, dims=(122, 60, 20), n_steps=24)
+    c = border_voxel_scene(90, dims=(60, 122, 20), n_steps=24)
+    with Solver() as s:
+        s.set_time_levels(2, 8)
+        for sc in (a, b, c):
+            pos, mat, _ = oracle.classify(sc.labels)
+            sig = sc.signals.astype(np.float32)
+            ref = oracle.run(pos, mat, sc.material_table(), sc.src_xyz, sc.src_kind, sig,
+                             sc.rec_xyz, sc.n_steps)
+            s.set_domain_labels(sc.labels)
+            s.set_materials(sc.material_table())
+            s.set_sources(sc.src_xyz, sc.src_kind, signal_f64=sc.signals, signal_f32=sig)
+            s.set_receivers(sc.rec_xyz)
+            s.run(sc.n_steps, N.F32)
+            assert np.array_equal(s.responses(np.float32), ref)
