@@ -13,9 +13,10 @@
 // the wave marches z with P^n of three slices and P^{n+1} of three slices in registers, the
 // second time level trailing the first by one slice. Every field and byte access of a slice is a
 // buffer instruction (descriptor of the slice in scalar registers, loop-invariant 32-bit lane
-// offsets), and all loads of a slice are issued before any is consumed. The four waves of a
-// block step z together (one barrier per slice) so that their overlapping halo rows meet in
-// cache, and the boundary coefficients are gathered from the block's LDS copy of the table.
+// offsets), and all loads of a slice are issued before any is consumed. A block's wave tiles
+// (4 x 4 for FP32, a column of 4 otherwise) step z together (one barrier per slice) so that
+// their overlapping halo columns and rows meet in cache, and the boundary coefficients are
+// gathered from the block's LDS copy of the table.
 // Rooms are mostly air: an air-row plane (one bit per slice, x tile and row, 1/480 of the
 // packed plane's bytes) marks the tile rows whose voxels are all plain air, and a wave whose
 // P^{n+1} rows of a slice are all marked takes the air value instead of loading the packed
@@ -37,7 +38,6 @@
 namespace pfdtd {
 
 constexpr int kFO = 60;       // owned columns per wave tile
-constexpr int kFWaves = 4;    // wave tiles per block, stacked in y
 constexpr int kFListCap = 32; // per-wave list capacity for sources / receivers (LDS)
 
 __device__ __forceinline__ bool in_box(const int32_t *q, int x0, int x1, int y0, int y1, int z0,
@@ -119,23 +119,38 @@ __device__ __forceinline__ unsigned pm_of(const uint32_t (&b)[NB], int j) {
   return (b[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
 }
 
-// Owned rows per wave tile (must be even: the byte planes of two rows share a register).
-// Measured on the 2048x1536x768 room: 2-byte fields 8 (120 VGPRs, four waves per SIMD; 11 %
-// faster than 10 rows at three waves, 10 % faster than 6 rows at five). FP32 6, the most that
-// keeps the kernel within 128 VGPRs (four waves per SIMD; 3 % faster than 4 rows at five).
-// FP64: 4.
+// Owned rows per wave tile TY (even: the byte planes of two rows share a register) and the
+// block's WX x WY wave tiles (x-fastest), which step z together so that the halo columns and
+// rows they share meet in cache. Measured on the 2048x1536x768 room:
+// * FP32: 6 rows, the most that keeps the kernel within 128 VGPRs (four waves per SIMD), in
+//   4 x 4 blocks (1024 threads, one block per CU): 18 % faster than 1 x 4 blocks, 4 % faster
+//   than 4 x 2, 8 % faster than 2 x 4.
+// * 2-byte fields: 8 rows (120 VGPRs, four waves per SIMD; 11 % faster than 10 rows at three
+//   waves, 10 % faster than 6 rows at five) in 1 x 4 blocks: the 16-bit sweep is not bound by
+//   HBM traffic, and every wider block (2 x 2, 2 x 4, 4 x 2, 4 x 4) is 1-24 % slower.
+// * FP64: 4 rows in 1 x 4 blocks (155 VGPRs).
 template <typename S>
 struct FusedTile {
-  static constexpr int TY = 8;
+  static constexpr int TY = 8, WX = 1, WY = 4;
 };
 template <>
 struct FusedTile<float> {
-  static constexpr int TY = 6;
+  static constexpr int TY = 6, WX = 4, WY = 4;
 };
 template <>
 struct FusedTile<double> {
-  static constexpr int TY = 4;
+  static constexpr int TY = 4, WX = 1, WY = 4;
 };
+// Serving-pool launches cover many small rooms (a few tiles each): they keep 1 x 4 blocks, so
+// that few waves of a block fall past a small grid.
+template <typename S, bool POOL>
+struct FusedShape {
+  static constexpr int WX = POOL ? 1 : FusedTile<S>::WX, WY = POOL ? 4 : FusedTile<S>::WY;
+};
+template <typename S, bool POOL>
+constexpr int fused_waves() {
+  return FusedShape<S, POOL>::WX * FusedShape<S, POOL>::WY;
+}
 
 // Kernel arguments: one context's domain, or the two-level jobs of one serving-pool launch.
 template <typename S, typename CT, typename C>
@@ -151,10 +166,11 @@ struct Fused2Pool {
 };
 
 template <typename S, typename CT, typename C, bool POOL>
-__global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
+__global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_kernel(
     typename std::conditional<POOL, Fused2Pool, Fused2Ctx<S, CT, C>>::type p) {
 #pragma clang fp contract(off)
   constexpr int kFTY = FusedTile<S>::TY;
+  constexpr int WX = FusedShape<S, POOL>::WX, WY = FusedShape<S, POOL>::WY, kFWaves = WX * WY;
   static_assert(kFTY % 2 == 0, "byte planes of two rows share one register");
   constexpr int NA = kFTY + 4;   // P^n rows per wave tile (2-voxel y halo)
   constexpr int NQ = kFTY + 2;   // P^{n+1} rows (1-voxel y halo)
@@ -236,9 +252,10 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
     byi = (t / J.fx) % J.fy;
     bzi = t / (J.fx * J.fy);
   }
-  const int xb = bxi * kFO;  // first owned column
+  const int xt = bxi * WX + wave % WX;  // this wave's x tile
+  const int xb = xt * kFO;  // first owned column
   const int x = xb - 2 + lane;      // this lane's column
-  const int yt = (byi * kFWaves + wave) * kFTY;  // first owned row
+  const int yt = (byi * WY + wave / WX) * kFTY;  // first owned row
   const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);  // owned slices [z0, z1)
 
@@ -313,10 +330,11 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
   const uint32_t *__restrict__ gar = a.air_rows;
   const int nyw = Y / 32 + 2;
   const size_t ar_slice = (size_t)((X + kFO - 1) / kFO) * (size_t)nyw;
-  const uint32_t *ar_tile = gar ? gar + (size_t)bxi * nyw + (yt >> 5) : nullptr;
+  const bool past_grid = yt >= Y || xb >= X;
+  const uint32_t *ar_tile = gar && !past_grid ? gar + (size_t)xt * nyw + (yt >> 5) : nullptr;
   auto air_window = [&](int k) -> uint64_t {
     if (!gar || k < 1 || k > Z - 2) return 0;
-    if (yt >= Y) return ~0ull;
+    if (past_grid) return ~0ull;
     const ConstU32 w = (ConstU32)(ar_tile + (size_t)k * ar_slice);
     return (((uint64_t)w[1] << 32) | w[0]) >> (yt & 31);
   };
@@ -328,7 +346,7 @@ __global__ __launch_bounds__(64 * kFWaves) void srl_update2_kernel(
   // 1..NQ at index row-1); bp/bc: packed bytes at k-1 and k (rows 1..NQ).
   auto body = [&](C (&am)[NA], C (&ac)[NA], C (&ap)[NA], C (&q0)[NQ], C (&q1)[NQ], C (&q2)[NQ],
                   uint32_t (&bp)[NB], uint32_t (&bc)[NB], int k) {
-    // the block's four waves load overlapping halo rows: keeping them on the same slice lets
+    // the block's waves load overlapping halo rows and columns: keeping them on the same slice lets
     // the cache serve the overlap once
     __builtin_amdgcn_s_barrier();
     const bool upd1 = k >= 1 && k <= Z - 2;
@@ -529,7 +547,7 @@ hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int
   return hipGetLastError();
 }
 
-FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision) {
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool) {
   const int kFTY = precision == 0 ? FusedTile<float>::TY
                                   : (precision == 1 ? FusedTile<double>::TY : FusedTile<h16>::TY);
   FusedGeom g;
@@ -538,9 +556,15 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision) {
   g.Z = Z;
   const int zi = Z - 2;
   g.zc = zc < 1 ? 1 : (zc > zi ? zi : zc);
-  g.grid = dim3((X + kFO - 1) / kFO, (Y + kFTY * kFWaves - 1) / (kFTY * kFWaves),
-                (zi + g.zc - 1) / g.zc);
-  g.block = dim3(64 * kFWaves, 1, 1);
+  const int WX = pool ? FusedShape<float, true>::WX
+                      : (precision == 0 ? FusedTile<float>::WX
+                                        : (precision == 1 ? FusedTile<double>::WX : FusedTile<h16>::WX));
+  const int WY = pool ? FusedShape<float, true>::WY
+                      : (precision == 0 ? FusedTile<float>::WY
+                                        : (precision == 1 ? FusedTile<double>::WY : FusedTile<h16>::WY));
+  const int ntx = (X + kFO - 1) / kFO;
+  g.grid = dim3((ntx + WX - 1) / WX, (Y + kFTY * WY - 1) / (kFTY * WY), (zi + g.zc - 1) / g.zc);
+  g.block = dim3(64 * WX * WY, 1, 1);
   return g;
 }
 
@@ -558,16 +582,19 @@ hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
                                hipStream_t stream) {
   const Fused2Pool p{jobs, st, L, zc};
   const size_t lds = sizeof(CoefF) * kStageRows * (size_t)L.ncol_max;
-  const dim3 grid((unsigned)tiles), block(64 * kFWaves);
+  const dim3 grid((unsigned)tiles);
   switch (precision) {
   case PFDTD_F32:
-    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true>), grid, block, lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true>), grid,
+                       dim3(64 * fused_waves<float, true>()), lds, stream, p);
     break;
   case PFDTD_F16:
-    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true>), grid, block, lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true>), grid,
+                       dim3(64 * fused_waves<h16, true>()), lds, stream, p);
     break;
   case PFDTD_BF16:
-    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true>), grid, block, lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true>), grid,
+                       dim3(64 * fused_waves<b16, true>()), lds, stream, p);
     break;
   default:
     return hipErrorInvalidValue;

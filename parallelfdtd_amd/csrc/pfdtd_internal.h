@@ -189,7 +189,8 @@ struct FusedGeom {
   int zc;
   dim3 grid, block;
 };
-FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision);
+// Grid of the two-level sweep; `pool`: the block shape of serving-pool launches.
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false);
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.

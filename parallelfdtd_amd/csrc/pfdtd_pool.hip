@@ -692,7 +692,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   d.gx = (X + kPoolTW - 1) / kPoolTW;
   d.gy = (Y + rows_per_tile - 1) / rows_per_tile;
   const int gz = (Z - 2 + p->zc - 1) / p->zc;
-  const FusedGeom fg = plan_fused(X, Y, Z, kPoolFusedZc, j->precision == PFDTD_F32 ? 0 : 2);
+  const FusedGeom fg = plan_fused(X, Y, Z, kPoolFusedZc, j->precision == PFDTD_F32 ? 0 : 2, true);
   d.fx = (int)fg.grid.x;
   d.fy = (int)fg.grid.y;
   d.fused = fused ? 1 : 0;
