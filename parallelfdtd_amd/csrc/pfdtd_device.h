@@ -40,8 +40,9 @@ struct Fmt<h16> {
   static __device__ __forceinline__ uint16_t bits(float v) {
     // The value must be rounded to FP32 first and then to binary16 (round to nearest even):
     // without the register barrier the compiler folds a producing multiply into
-    // v_fma_mixlo_f16, which rounds the exact product once, straight to binary16.
-    asm volatile("" : "+v"(v));
+    // v_fma_mixlo_f16, which rounds the exact product once, straight to binary16. The barrier
+    // is not volatile, so the compiler may still evaluate it speculatively (branch-free selects).
+    asm("" : "+v"(v));
     return __builtin_bit_cast(uint16_t, (_Float16)v);
   }
   static __device__ __forceinline__ void st(h16 *p, int64_t i, float v) { p[i].bits = bits(v); }

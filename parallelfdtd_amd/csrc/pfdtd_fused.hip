@@ -62,23 +62,24 @@ __device__ __forceinline__ int wave_collect(const int32_t *xyz, int n, int x0, i
   return cnt;
 }
 
-// x-neighbours across the wavefront with DPP row shifts: wave_shl:1 gives lane i the value of
-// lane i+1, wave_shr:1 the value of lane i-1 (lanes without a source get 0; they are halo
-// lanes whose results are never used).
+// x-neighbours across the wavefront with DPP shifts: wave_shl:1 gives lane i the value of
+// lane i+1, wave_shr:1 the value of lane i-1; lanes without a source read 0 (bound_ctrl; they
+// are halo lanes whose results are never used). With no `old` operand to keep, the compiler
+// folds each shift into the add that consumes it (v_add_f32_dpp).
 __device__ __forceinline__ float lane_next(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float lane_prev(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double lane_next(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double lane_prev(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -165,8 +166,14 @@ struct Fused2Pool {
   int zc;
 };
 
+// Four waves per SIMD (at most 128 VGPRs) for FP32 and 16-bit storage; FP64 needs more.
+template <typename S>
+constexpr int fused_min_waves() {
+  return sizeof(S) == 8 ? 1 : 4;
+}
+
 template <typename S, typename CT, typename C, bool POOL>
-__global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_kernel(
+__global__ __launch_bounds__((64 * fused_waves<S, POOL>()), (fused_min_waves<S>())) void srl_update2_kernel(
     typename std::conditional<POOL, Fused2Pool, Fused2Ctx<S, CT, C>>::type p) {
 #pragma clang fp contract(off)
   constexpr int kFTY = FusedTile<S>::TY;
@@ -282,31 +289,40 @@ __global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_ker
   const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^32
   const int xc = min(max(x, 0), X - 1);
   // Rows r = 0..NA-1 of the P^n block are y = yt - 2 + r (clamped to the grid for loads).
+  // Node (x, y_r) is updated when its column and its row are off the outer layer, and stored
+  // when the lane owns its column and its row lies in the grid. The row parts are wave-uniform
+  // bit masks (scalar registers), the column parts one lane condition each, so the per-row
+  // conditions cost no 64-bit lane masks kept across the loop.
   uint32_t ro[NA];
-  uint32_t imask = 0;  // bit r: node (x, y_r) is updated (not on the outer layer)
+  uint32_t row_upd = 0;  // bit r: row y = yt - 2 + r is off the outer layer
 #pragma unroll
   for (int r = 0; r < NA; ++r) {
     const int y = yt - 2 + r;
     ro[r] = (uint32_t)(min(max(y, 0), Y - 1) * X + xc);
-    if (x > 0 && x < X - 1 && y > 0 && y < Y - 1) imask |= 1u << r;
+    if (y > 0 && y < Y - 1) row_upd |= 1u << r;
   }
+  const bool col_upd = x > 0 && x < X - 1;
   const bool owned_lane = lane >= 2 && lane < 2 + kFO && x < X;
-  uint32_t smask = 0;  // bit o: owned row o (y = yt + o) is stored by this lane
+  uint32_t row_st = 0;  // bit o: owned row o (y = yt + o) lies in the grid
 #pragma unroll
   for (int o = 0; o < kFTY; ++o)
-    if (owned_lane && yt + o < Y) smask |= 1u << o;
+    if (yt + o < Y) row_st |= 1u << o;
   const double inv_scale = ldexp(1.0, -a.scale_exp);
   const C scale = (C)ldexp(1.0, a.scale_exp);
 
   const typename CoefOf<C>::T air = a.air;
   const C lam2 = a.lam2;
-  // TABLE: gather the (K, material) entry; otherwise every voxel of the stage is air.
+  // TABLE: gather the (K, material) entry. Otherwise every position byte of the stage is 0x86
+  // (air, inside: the factor pb >> 7 is 1 and 1 * inv == inv) and every K = 6 entry is `air`.
   auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
-    const unsigned pb = pm & 0xFFu;
-    typename CoefOf<C>::T e = air;
-    if (decltype(table)::value) e = s_coef[min(pb & 0x7Fu, 6u) * ncol + (pm >> 8)];
-    const C aa = (C)(pb >> 7) * e.inv;
-    return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
+    if constexpr (decltype(table)::value) {
+      const unsigned pb = pm & 0xFFu;
+      const typename CoefOf<C>::T e = s_coef[min(pb & 0x7Fu, 6u) * ncol + (pm >> 8)];
+      const C aa = (C)(pb >> 7) * e.inv;
+      return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
+    } else {
+      return air.inv * ((air.tk * pc + lam2 * s) - air.omb * past);
+    }
   };
   const bool air_uniform = a.air_uniform != 0;
   // wave-uniform: does any of the NQ packed rows hold a voxel other than air (0x86)?
@@ -346,6 +362,11 @@ __global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_ker
   // 1..NQ at index row-1); bp/bc: packed bytes at k-1 and k (rows 1..NQ).
   auto body = [&](C (&am)[NA], C (&ac)[NA], C (&ap)[NA], C (&q0)[NQ], C (&q1)[NQ], C (&q2)[NQ],
                   uint32_t (&bp)[NB], uint32_t (&bc)[NB], int k) {
+    // fresh copies each slice: the per-row conditions derived from them are not hoisted out of
+    // the loop as lane masks
+    uint32_t ru = __builtin_amdgcn_readfirstlane(row_upd);
+    uint32_t rs = __builtin_amdgcn_readfirstlane(row_st);
+    asm volatile("" : "+s"(ru), "+s"(rs));
     // the block's waves load overlapping halo rows and columns: keeping them on the same slice lets
     // the cache serve the overlap once
     __builtin_amdgcn_s_barrier();
@@ -396,8 +417,8 @@ __global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_ker
         s = s + ac[r - 1];                   // y-1
         s = s + lane_next(ac[r]);            // x+1
         s = s + lane_prev(ac[r]);            // x-1
-        const C v = node(table, pm_of(bc, j), ac[r], s, past[j]);
-        q2[j] = (upd1 && ((imask >> r) & 1u)) ? Fmt<S>::round(v) : (C)0;
+        const C v = Fmt<S>::round(node(table, pm_of(bc, j), ac[r], s, past[j]));
+        q2[j] = (upd1 && ((ru >> r) & 1u) && col_upd) ? v : (C)0;
       }
     };
     if (air1 ? !air_uniform : any_nonair(bc))
@@ -446,7 +467,7 @@ __global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_ker
       const auto rC = slice_rsrc(gC + (size_t)k * XY, sbytes);
 #pragma unroll
       for (int o = 0; o < kFTY; ++o)
-        if ((smask >> o) & 1u) bst<S>(rC, ro[o + 2], Fmt<S>::raw(q2[o + 1]));
+        if (((rs >> o) & 1u) && owned_lane) bst<S>(rC, ro[o + 2], Fmt<S>::raw(q2[o + 1]));
     }
 
     // ---- second time level: P^{n+2}[k-1] on the owned rows
@@ -462,8 +483,8 @@ __global__ __launch_bounds__((64 * fused_waves<S, POOL>())) void srl_update2_ker
           s = s + lane_next(q1[j]);          // x+1
           s = s + lane_prev(q1[j]);          // x-1
           C v = node(table, pm_of(bp, j), q1[j], s, am[o + 2]);
-          v = ((imask >> (o + 2)) & 1u) ? v : (C)0;
-          if ((smask >> o) & 1u) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
+          v = (((ru >> (o + 2)) & 1u) && col_upd) ? v : (C)0;
+          if (((rs >> o) & 1u) && owned_lane) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
         }
       };
       if (any_nonair(bp))
