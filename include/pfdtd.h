@@ -49,6 +49,11 @@ extern "C" {
 #define PFDTD_SOFT_ADD 0         /* soft source adds (CudaMeshTest.cpp:311-314) */
 #define PFDTD_SOFT_OVERWRITE 2   /* soft source overwrites, as addSample does (cudaMesh.h:362-366) */
 
+/* ---- block shape of the two-level sweep (pfdtd_set_sweep_shape) -------------------- */
+#define PFDTD_SWEEP_AUTO 0   /* wide for FP32 rooms of at least 960 columns, else column */
+#define PFDTD_SWEEP_COLUMN 1 /* blocks of 1 x 4 wave tiles (60 columns x 4 tiles of rows) */
+#define PFDTD_SWEEP_WIDE 2   /* FP32: blocks of 4 x 4 wave tiles (240 columns, 1024 threads) */
+
 typedef struct pfdtd_ctx pfdtd_ctx;
 
 /* InterruptCallback bool(*)(void), src/App.h:33 -- nonzero stops the run. */
@@ -231,6 +236,12 @@ int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
  * Results are bit-identical to single steps. No reference counterpart: the reference's loop
  * (kernels3d.cu:83-191) launches one update per step. */
 int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
+/* Block shape of the two-level sweep: PFDTD_SWEEP_AUTO (default), _COLUMN or _WIDE (FP32 only;
+ * other precisions always use columns). The wave tiles of a block step z together so their
+ * shared halo columns and rows meet in cache: wide blocks share more of them but leave up to
+ * three tiles past the grid's last column, so they pay off on wide rooms. Results do not
+ * depend on it. No reference counterpart (a tuning knob of the two-level sweep). */
+int pfdtd_set_sweep_shape(pfdtd_ctx *ctx, int shape);
 /* Update-kernel tiling: y rows per thread (1, 2, 4 or 8; default 2) and slices per z-chunk
  * (default 8). Results do not depend on it (every voxel's arithmetic is the same); it only
  * changes the streaming pattern. Replaces the fixed 32x4 block of App::initializeMesh

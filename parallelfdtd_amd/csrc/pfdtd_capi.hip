@@ -97,6 +97,7 @@ struct pfdtd_ctx {
   int zc = 8;             // slices per z-chunk of the update kernel
   int levels = 0;         // time levels per sweep: 1, 2, or 0 = by precision
   int fzc = 32;           // slices per z-chunk of the two-level update
+  int fshape = PFDTD_SWEEP_AUTO;  // block shape of the two-level update
 
   // sources / receivers
   int n_src = 0, n_samples = 0;
@@ -1061,7 +1062,8 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
       c->pm_ok = true;
     }
   }
-  const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision);
+  const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
+                                  c->fshape);
   int done = 0;
   auto wall0 = std::chrono::steady_clock::now();
   while (done < n_steps) {
@@ -1287,6 +1289,14 @@ int pfdtd_set_time_levels(pfdtd_ctx *c, int levels, int zchunk) {
   }
   c->levels = levels;
   c->fzc = zchunk;
+  return PFDTD_OK;
+}
+
+int pfdtd_set_sweep_shape(pfdtd_ctx *c, int shape) {
+  if (!c) return fail(PFDTD_E_ARG, "null context");
+  if (shape != PFDTD_SWEEP_AUTO && shape != PFDTD_SWEEP_COLUMN && shape != PFDTD_SWEEP_WIDE)
+    return fail(PFDTD_E_ARG, "sweep shape %d not AUTO (0), COLUMN (1) or WIDE (2)", shape);
+  c->fshape = shape;
   return PFDTD_OK;
 }
 

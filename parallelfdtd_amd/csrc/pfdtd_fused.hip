@@ -14,9 +14,9 @@
 // second time level trailing the first by one slice. Every field and byte access of a slice is a
 // buffer instruction (descriptor of the slice in scalar registers, loop-invariant 32-bit lane
 // offsets), and all loads of a slice are issued before any is consumed. A block's wave tiles
-// (4 x 4 for FP32, a column of 4 otherwise) step z together (one barrier per slice) so that
-// their overlapping halo columns and rows meet in cache, and the boundary coefficients are
-// gathered from the block's LDS copy of the table.
+// (4 x 4 for FP32 on wide rooms, a column of 4 otherwise) step z together (one barrier per
+// slice) so that their overlapping halo columns and rows meet in cache, and the boundary
+// coefficients are gathered from the block's LDS copy of the table.
 // Rooms are mostly air: an air-row plane (one bit per slice, x tile and row, 1/480 of the
 // packed plane's bytes) marks the tile rows whose voxels are all plain air, and a wave whose
 // P^{n+1} rows of a slice are all marked takes the air value instead of loading the packed
@@ -120,37 +120,37 @@ __device__ __forceinline__ unsigned pm_of(const uint32_t (&b)[NB], int j) {
   return (b[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
 }
 
-// Owned rows per wave tile TY (even: the byte planes of two rows share a register) and the
-// block's WX x WY wave tiles (x-fastest), which step z together so that the halo columns and
-// rows they share meet in cache. Measured on the 2048x1536x768 room:
-// * FP32: 6 rows, the most that keeps the kernel within 128 VGPRs (four waves per SIMD), in
-//   4 x 4 blocks (1024 threads, one block per CU): 18 % faster than 1 x 4 blocks, 4 % faster
-//   than 4 x 2, 8 % faster than 2 x 4.
-// * 2-byte fields: 8 rows (120 VGPRs, four waves per SIMD; 11 % faster than 10 rows at three
-//   waves, 10 % faster than 6 rows at five) in 1 x 4 blocks: the 16-bit sweep is not bound by
-//   HBM traffic, and every wider block (2 x 2, 2 x 4, 4 x 2, 4 x 4) is 1-24 % slower.
-// * FP64: 4 rows in 1 x 4 blocks (155 VGPRs).
+// Owned rows per wave tile TY (even: the byte planes of two rows share a register). Measured
+// on the 2048x1536x768 room: FP32 6, the most that keeps the kernel within 128 VGPRs (four waves
+// per SIMD; 3 % faster than 4 rows at five); 2-byte fields 8 (120 VGPRs, four waves per SIMD;
+// 11 % faster than 10 rows at three waves, 10 % faster than 6 rows at five); FP64 4.
 template <typename S>
 struct FusedTile {
-  static constexpr int TY = 8, WX = 1, WY = 4;
+  static constexpr int TY = 8;
 };
 template <>
 struct FusedTile<float> {
-  static constexpr int TY = 6, WX = 4, WY = 4;
+  static constexpr int TY = 6;
 };
 template <>
 struct FusedTile<double> {
-  static constexpr int TY = 4, WX = 1, WY = 4;
+  static constexpr int TY = 4;
 };
-// Serving-pool launches cover many small rooms (a few tiles each): they keep 1 x 4 blocks, so
-// that few waves of a block fall past a small grid.
-template <typename S, bool POOL>
+// A block holds WX x WY wave tiles (x-fastest) that step z together, so that the halo columns
+// and rows they share meet in cache: a column of 1 x 4 tiles, or (WIDE, FP32 only) 4 x 4 tiles,
+// 1024 threads covering 240 columns x 24 rows, one block per CU. On the 2048x1536x768 room the
+// wide FP32 block is 18 % faster than the column (4 x 2: 4 %, 2 x 4: 8 % slower than 4 x 4);
+// on the 512x384x256 room, with 3 of its 12 x tiles past the grid and few blocks per CU, it is
+// 7 % slower. The 16-bit sweep is not bound by HBM traffic: every wider block (2 x 2, 2 x 4,
+// 4 x 2, 4 x 4) was 1-24 % slower than the column. Serving-pool launches (small rooms) use the
+// column.
+template <bool WIDE>
 struct FusedShape {
-  static constexpr int WX = POOL ? 1 : FusedTile<S>::WX, WY = POOL ? 4 : FusedTile<S>::WY;
+  static constexpr int WX = WIDE ? 4 : 1, WY = 4;
 };
-template <typename S, bool POOL>
+template <bool WIDE>
 constexpr int fused_waves() {
-  return FusedShape<S, POOL>::WX * FusedShape<S, POOL>::WY;
+  return FusedShape<WIDE>::WX * FusedShape<WIDE>::WY;
 }
 
 // Kernel arguments: one context's domain, or the two-level jobs of one serving-pool launch.
@@ -172,12 +172,12 @@ constexpr int fused_min_waves() {
   return sizeof(S) == 8 ? 1 : 4;
 }
 
-template <typename S, typename CT, typename C, bool POOL>
-__global__ __launch_bounds__((64 * fused_waves<S, POOL>()), (fused_min_waves<S>())) void srl_update2_kernel(
+template <typename S, typename CT, typename C, bool POOL, bool WIDE>
+__global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>())) void srl_update2_kernel(
     typename std::conditional<POOL, Fused2Pool, Fused2Ctx<S, CT, C>>::type p) {
 #pragma clang fp contract(off)
   constexpr int kFTY = FusedTile<S>::TY;
-  constexpr int WX = FusedShape<S, POOL>::WX, WY = FusedShape<S, POOL>::WY, kFWaves = WX * WY;
+  constexpr int WX = FusedShape<WIDE>::WX, WY = FusedShape<WIDE>::WY, kFWaves = WX * WY;
   static_assert(kFTY % 2 == 0, "byte planes of two rows share one register");
   constexpr int NA = kFTY + 4;   // P^n rows per wave tile (2-voxel y halo)
   constexpr int NQ = kFTY + 2;   // P^{n+1} rows (1-voxel y halo)
@@ -568,7 +568,7 @@ hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int
   return hipGetLastError();
 }
 
-FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool) {
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int shape) {
   const int kFTY = precision == 0 ? FusedTile<float>::TY
                                   : (precision == 1 ? FusedTile<double>::TY : FusedTile<h16>::TY);
   FusedGeom g;
@@ -577,13 +577,13 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool) {
   g.Z = Z;
   const int zi = Z - 2;
   g.zc = zc < 1 ? 1 : (zc > zi ? zi : zc);
-  const int WX = pool ? FusedShape<float, true>::WX
-                      : (precision == 0 ? FusedTile<float>::WX
-                                        : (precision == 1 ? FusedTile<double>::WX : FusedTile<h16>::WX));
-  const int WY = pool ? FusedShape<float, true>::WY
-                      : (precision == 0 ? FusedTile<float>::WY
-                                        : (precision == 1 ? FusedTile<double>::WY : FusedTile<h16>::WY));
   const int ntx = (X + kFO - 1) / kFO;
+  // automatic: wide blocks for FP32 rooms at least 16 x tiles (960 columns) wide, where at
+  // most 3 of 16 tiles fall past the grid
+  g.wide = !pool && precision == 0 &&
+           (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16));
+  const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
+  const int WY = g.wide ? FusedShape<true>::WY : FusedShape<false>::WY;
   g.grid = dim3((ntx + WX - 1) / WX, (Y + kFTY * WY - 1) / (kFTY * WY), (zi + g.zc - 1) / g.zc);
   g.block = dim3(64 * WX * WY, 1, 1);
   return g;
@@ -593,8 +593,14 @@ template <typename S, typename CT, typename C>
 static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a,
                                hipStream_t stream) {
   const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
-  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false>), g.grid, g.block, lds, stream,
-                     Fused2Ctx<S, CT, C>{a, g.X, g.Y, g.Z, g.zc});
+  const Fused2Ctx<S, CT, C> p{a, g.X, g.Y, g.Z, g.zc};
+  if constexpr (std::is_same<S, float>::value) {
+    if (g.wide) {
+      hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, true>), g.grid, g.block, lds, stream, p);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false>), g.grid, g.block, lds, stream, p);
   return hipGetLastError();
 }
 
@@ -606,16 +612,16 @@ hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
   const dim3 grid((unsigned)tiles);
   switch (precision) {
   case PFDTD_F32:
-    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true>), grid,
-                       dim3(64 * fused_waves<float, true>()), lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true, false>), grid,
+                       dim3(64 * fused_waves<false>()), lds, stream, p);
     break;
   case PFDTD_F16:
-    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true>), grid,
-                       dim3(64 * fused_waves<h16, true>()), lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true, false>), grid,
+                       dim3(64 * fused_waves<false>()), lds, stream, p);
     break;
   case PFDTD_BF16:
-    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true>), grid,
-                       dim3(64 * fused_waves<b16, true>()), lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true, false>), grid,
+                       dim3(64 * fused_waves<false>()), lds, stream, p);
     break;
   default:
     return hipErrorInvalidValue;

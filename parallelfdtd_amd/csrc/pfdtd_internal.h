@@ -187,10 +187,13 @@ struct Fused2Args {
 struct FusedGeom {
   int X, Y, Z;
   int zc;
+  bool wide;  // 4 x 4-wave blocks (FP32), else 1 x 4
   dim3 grid, block;
 };
-// Grid of the two-level sweep; `pool`: the block shape of serving-pool launches.
-FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false);
+// Grid of the two-level sweep; `pool`: serving-pool launches (1 x 4 blocks); `shape`: one of
+// PFDTD_SWEEP_AUTO / _COLUMN / _WIDE.
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false,
+                     int shape = 0);
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.

@@ -17,6 +17,7 @@ OK = 0
 E_ARG, E_HIP, E_NOMEM, E_STATE, E_DOMAIN = -1, -2, -3, -4, -5
 
 F32, F64, F16, BF16 = 0, 1, 2, 3
+SWEEP_AUTO, SWEEP_COLUMN, SWEEP_WIDE = 0, 1, 2
 SRC_HARD, SRC_SOFT, SRC_TRANSPARENT = 0, 1, 2
 SRL_FORWARD, SHARED, SRL = 0, 1, 2  # enum UpdateType
 MATSLOT_INTENDED, MATSLOT_LITERAL = 0, 1
@@ -81,6 +82,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_receiver_slots": [_p, C.c_int, _p],
     "pfdtd_set_update_tiling": [_p, C.c_int, C.c_int],
     "pfdtd_set_time_levels": [_p, C.c_int, C.c_int],
+    "pfdtd_set_sweep_shape": [_p, C.c_int],
     "pfdtd_pool_create": [C.c_int, _i64, C.POINTER(_p)],
     "pfdtd_pool_destroy": [_p],
     "pfdtd_pool_set_options": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],

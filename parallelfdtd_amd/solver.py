@@ -188,6 +188,10 @@ class Solver:
     def set_time_levels(self, levels: int = 0, zchunk: int = 32) -> None:
         self._c("pfdtd_set_time_levels", levels, zchunk)
 
+    def set_sweep_shape(self, shape: int = N.SWEEP_AUTO) -> None:
+        """Block shape of the two-level sweep: N.SWEEP_AUTO, N.SWEEP_COLUMN or N.SWEEP_WIDE."""
+        self._c("pfdtd_set_sweep_shape", shape)
+
     def set_poll_interval(self, steps: int) -> None:
         self._c("pfdtd_set_poll_interval", steps)
 

@@ -169,3 +169,18 @@ def test_batch_argument_checks(gpu):
             s.set_source_slots([0])         # wrong count
         with pytest.raises(N.PfdtdError):
             s.set_receiver_slots([-1, 0])
+
+
+def test_autotune_sweep_small_room(gpu):
+    """The tuner times single-level candidates for every B and the two-level path for B = 1,
+    and chooses the fastest per source."""
+    from parallelfdtd_amd import autotune
+    scene = rooms.scene_c4(n_sources=8, n_steps=6, scale=8)
+    rep = autotune.sweep(scene, batches=(1, 8), rows=(2,), zchunks=(8,), steps=6, warmup=1,
+                         log=lambda m: None)
+    levels = sorted((c["B"], c["levels"], c["shape"] or "") for c in rep["candidates"])
+    assert levels == [(1, 1, "")] + [(1, 2, "column")] * 3 + [(1, 2, "wide")] * 3 + [(8, 1, "")]
+    assert set(rep["best"]) == {"1", "8"} and set(rep["best_single_level"]) == {"1", "8"}
+    assert rep["best_single_level"]["1"]["levels"] == 1
+    top = max(rep["candidates"], key=lambda c: c["per_source_gvox"])
+    assert rep["choice"]["B"] == top["B"] and rep["choice"]["levels"] == top["levels"]

@@ -17,7 +17,7 @@ PREC = {N.F32: ("f32", 0, 0, np.float32), N.F64: ("f64", 0, 0, np.float64),
         N.F16: ("f32", oracle.STORE_F16, 9, np.float16), N.BF16: ("f32", oracle.STORE_BF16, 9, np.uint16)}
 
 
-def run_both(sc, prec, levels=2, zchunk=32, literal=0, octave=0, poll=100):
+def run_both(sc, prec, levels=2, zchunk=32, literal=0, octave=0, poll=100, shape=N.SWEEP_AUTO):
     pname, storage, scale, raw = PREC[prec]
     pos, mat, _ = oracle.classify(sc.labels)
     sig = sc.signals if pname == "f64" else sc.signals.astype(np.float32)
@@ -27,6 +27,7 @@ def run_both(sc, prec, levels=2, zchunk=32, literal=0, octave=0, poll=100):
                                     want_fields=True)
     with Solver() as s:
         s.set_time_levels(levels, zchunk)
+        s.set_sweep_shape(shape)
         s.set_domain_labels(sc.labels)
         s.set_materials(sc.material_table(), octave=octave, literal_slot=bool(literal & 1),
                         soft_overwrite=bool(literal & 2))
@@ -191,3 +192,27 @@ def test_two_level_domain_reset_with_same_voxel_count(gpu):
             s.set_receivers(sc.rec_xyz)
             s.run(sc.n_steps, N.F32)
             assert np.array_equal(s.responses(np.float32), ref)
+
+
+@pytest.mark.parametrize("shape", [N.SWEEP_COLUMN, N.SWEEP_WIDE])
+@pytest.mark.parametrize("prec", [N.F32, N.F16])
+def test_two_level_block_shapes(gpu, shape, prec):
+    """Both block shapes on a room whose 4 x tiles leave wide blocks partly past the grid (the
+    16-bit sweep always uses columns), with boundary voxels on the tile borders."""
+    sc = border_voxel_scene(91, dims=(250, 54, 20), n_steps=30)
+    assert_same(*run_both(sc, prec, zchunk=7, shape=shape))
+
+
+def test_two_level_wide_room_automatic_shape(gpu):
+    """A room of 1000 columns (17 x tiles) takes wide blocks by default; 3 of its 20 block tiles
+    lie past the last column."""
+    sc = rooms.scene_random_small(92, dims=(1000, 30, 12), n_mat=3, n_boxes=5, n_src=6, n_rec=8,
+                                  n_steps=26)
+    assert_same(*run_both(sc, N.F32, zchunk=4))
+    assert_same(*run_both(sc, N.F32, zchunk=4, shape=N.SWEEP_COLUMN))
+
+
+def test_sweep_shape_rejects_unknown(gpu):
+    with Solver() as s:
+        with pytest.raises(N.PfdtdError):
+            s.set_sweep_shape(3)
