@@ -96,7 +96,7 @@ struct pfdtd_ctx {
   int ry = 2;             // y rows per thread of the update kernel
   int zc = 8;             // slices per z-chunk of the update kernel
   int levels = 0;         // time levels per sweep: 1, 2, or 0 = by precision
-  int fzc = 32;           // slices per z-chunk of the two-level update
+  int fzc = 0;            // slices per z-chunk of the two-level update (0: automatic)
   int fshape = PFDTD_SWEEP_AUTO;  // block shape of the two-level update
 
   // sources / receivers
@@ -1281,7 +1281,7 @@ int pfdtd_set_update_tiling(pfdtd_ctx *c, int rows, int zchunk) {
 int pfdtd_set_time_levels(pfdtd_ctx *c, int levels, int zchunk) {
   if (!c) return fail(PFDTD_E_ARG, "null context");
   if (levels < 0 || levels > 2) return fail(PFDTD_E_ARG, "time levels %d not in {0,1,2}", levels);
-  if (zchunk < 1 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 1..65535", zchunk);
+  if (zchunk < 0 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 0..65535", zchunk);
   if (levels != c->levels) {
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -576,7 +576,6 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   g.Y = Y;
   g.Z = Z;
   const int zi = Z - 2;
-  g.zc = zc < 1 ? 1 : (zc > zi ? zi : zc);
   const int ntx = (X + kFO - 1) / kFO;
   // automatic: wide blocks for FP32 rooms at least 16 x tiles (960 columns) wide, where at
   // most 3 of 16 tiles fall past the grid
@@ -584,6 +583,15 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
            (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16));
   const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
   const int WY = g.wide ? FusedShape<true>::WY : FusedShape<false>::WY;
+  const int64_t bxy = (int64_t)((ntx + WX - 1) / WX) * ((Y + kFTY * WY - 1) / (kFTY * WY));
+  if (zc < 1) {
+    // automatic: the longest of 32, 16 and 8 slices that still gives 4096 blocks (16 per CU),
+    // so small rooms keep every CU busy to the end of the sweep; each chunk re-reads one slice
+    // of P^n on either side, so longer chunks move fewer bytes
+    zc = 32;
+    while (zc > 8 && bxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
+  }
+  g.zc = zc > zi ? zi : zc;
   g.grid = dim3((ntx + WX - 1) / WX, (Y + kFTY * WY - 1) / (kFTY * WY), (zi + g.zc - 1) / g.zc);
   g.block = dim3(64 * WX * WY, 1, 1);
   return g;

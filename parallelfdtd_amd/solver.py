@@ -185,7 +185,7 @@ class Solver:
         sl = np.ascontiguousarray(slots, np.int32)
         self._c("pfdtd_set_receiver_slots", sl.size, N.ptr(sl))
 
-    def set_time_levels(self, levels: int = 0, zchunk: int = 32) -> None:
+    def set_time_levels(self, levels: int = 0, zchunk: int = 0) -> None:
         self._c("pfdtd_set_time_levels", levels, zchunk)
 
     def set_sweep_shape(self, shape: int = N.SWEEP_AUTO) -> None:

@@ -216,3 +216,14 @@ def test_sweep_shape_rejects_unknown(gpu):
     with Solver() as s:
         with pytest.raises(N.PfdtdError):
             s.set_sweep_shape(3)
+
+
+@pytest.mark.parametrize("prec", [N.F32, N.F16])
+def test_two_level_automatic_zchunk(gpu, prec):
+    """z-chunk 0 (the default) picks the chunk length from the grid; results do not change."""
+    sc = rooms.scene_random_small(93, dims=(130, 70, 41), n_mat=4, n_boxes=6, n_src=3, n_rec=6,
+                                  n_steps=33)
+    assert_same(*run_both(sc, prec, zchunk=0))
+    with Solver() as s:
+        with pytest.raises(N.PfdtdError):
+            s.set_time_levels(2, -1)
