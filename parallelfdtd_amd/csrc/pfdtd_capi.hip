@@ -405,7 +405,7 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
     return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
   if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
   const size_t bytes = es * ((size_t)c->N << c->lgB);
-  const bool want4 = c->lgB == 0 && c->scheme == 0 && levels_for(c, precision) == 2;
+  const bool want4 = c->lgB == 0 && levels_for(c, precision) == 2;
   if (c->d_P && c->field_bytes == bytes && (!want4 || c->d_P2)) {
     c->precision = precision;
     c->scale_exp = scale_exp;
@@ -508,7 +508,7 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
 // The two-level update needs the spare pair and no source on the never-updated slices
 // z = 0 and z = Z-1 (their values would have to be carried between the four buffers).
 bool fused_eligible(const pfdtd_ctx *c) {
-  if (c->scheme != 0 || c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
+  if (c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
     return false;
   // the two-level kernel addresses each z-slice through a buffer descriptor (32-bit range)
   const uint64_t esz = c->precision == PFDTD_F64 ? 8 : (c->precision == PFDTD_F32 ? 4 : 2);
@@ -1055,7 +1055,8 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
         c->d_air_rows = nullptr;
       }
       if (c->d_air_rows &&
-          launch_air_rows(c->d_pm, c->d_air_rows, (int)c->X, (int)c->Y, (int)c->Z, c->stream) != hipSuccess) {
+          launch_air_rows(c->d_pm, c->d_air_rows, (int)c->X, (int)c->Y, (int)c->Z, c->scheme,
+                          c->stream) != hipSuccess) {
         (void)hipGetLastError();
         dfree(c->d_air_rows);
       }
@@ -1063,7 +1064,7 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
     }
   }
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
-                                  c->fshape);
+                                  c->fshape, c->scheme);
   int done = 0;
   auto wall0 = std::chrono::steady_clock::now();
   while (done < n_steps) {

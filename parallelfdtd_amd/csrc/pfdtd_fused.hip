@@ -136,6 +136,17 @@ template <>
 struct FusedTile<double> {
   static constexpr int TY = 4;
 };
+// The centred update keeps six separate neighbours per node: 2-byte fields take 6-row tiles
+// there (110-113 VGPRs, four waves per SIMD; 8 rows need 236).
+template <typename S, int SCHEME>
+constexpr int fused_ty() {
+  return (SCHEME == 2 && sizeof(S) == 2) ? 6 : FusedTile<S>::TY;
+}
+static int fused_ty_host(int precision, int scheme) {
+  if (precision == 0) return fused_ty<float, 0>();
+  if (precision == 1) return fused_ty<double, 0>();
+  return scheme == 2 ? fused_ty<h16, 2>() : fused_ty<h16, 0>();
+}
 // A block holds WX x WY wave tiles (x-fastest) that step z together, so that the halo columns
 // and rows they share meet in cache: a column of 1 x 4 tiles, or (WIDE, FP32 only) 4 x 4 tiles,
 // 1024 threads covering 240 columns x 24 rows, one block per CU. On the 2048x1536x768 room the
@@ -166,17 +177,21 @@ struct Fused2Pool {
   int zc;
 };
 
-// Four waves per SIMD (at most 128 VGPRs) for FP32 and 16-bit storage; FP64 needs more.
-template <typename S>
+// Four waves per SIMD (at most 128 VGPRs) for the forward update with FP32 and 16-bit storage;
+// FP64 and the centred update (six separate neighbours per node) need more registers.
+template <typename S, int SCHEME>
 constexpr int fused_min_waves() {
-  return sizeof(S) == 8 ? 1 : 4;
+  return (sizeof(S) == 8 || SCHEME == 2) ? 1 : 4;
 }
 
-template <typename S, typename CT, typename C, bool POOL, bool WIDE>
-__global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>())) void srl_update2_kernel(
+// SCHEME 0: forward (Bilbao) boundaries, update types SRL_FORWARD and SHARED; SCHEME 2: the
+// centred (Kowalczyk) boundaries of update type SRL, through the same `centred_node` as the
+// single-step kernel (the reference's operation order).
+template <typename S, typename CT, typename C, bool POOL, bool WIDE, int SCHEME>
+__global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCHEME>())) void srl_update2_kernel(
     typename std::conditional<POOL, Fused2Pool, Fused2Ctx<S, CT, C>>::type p) {
 #pragma clang fp contract(off)
-  constexpr int kFTY = FusedTile<S>::TY;
+  constexpr int kFTY = fused_ty<S, SCHEME>();
   constexpr int WX = FusedShape<WIDE>::WX, WY = FusedShape<WIDE>::WY, kFWaves = WX * WY;
   static_assert(kFTY % 2 == 0, "byte planes of two rows share one register");
   constexpr int NA = kFTY + 4;   // P^n rows per wave tile (2-voxel y halo)
@@ -355,6 +370,8 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>()))
     return (((uint64_t)w[1] << 32) | w[0]) >> (yt & 31);
   };
   constexpr uint64_t kAllRows = (1ull << NQ) - 1ull;
+  // the packed value of plain air (position 0x86 forward, 0x80 centred; material 0), twice
+  constexpr uint32_t kAirPair = SCHEME == 2 ? 0x00800080u : 0x00860086u;
   uint64_t ar_win = air_window(z0 - 1);  // the window of the slice the next body call updates
 
   // One z-iteration: first level at slice k (rows 1..NQ), second level at slice k-1 (rows
@@ -389,7 +406,7 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>()))
       if (air1) {
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < NB; ++i) bc[i] = 0x00860086u;
+        for (int i = 0; i < NB; ++i) bc[i] = kAirPair;
       } else {
         const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
         uint32_t pmv[NQ];
@@ -421,10 +438,22 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>()))
         q2[j] = (upd1 && ((ru >> r) & 1u) && col_upd) ? v : (C)0;
       }
     };
-    if (air1 ? !air_uniform : any_nonair(bc))
+    if constexpr (SCHEME == 2) {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const int r = j + 1;
+        const unsigned pmv = pm_of(bc, j);
+        const C v = Fmt<S>::round(centred_node<C>(pmv & 0xFFu, pmv >> 8, ac[r], ap[r], am[r],
+                                                  ac[r + 1], ac[r - 1], lane_next(ac[r]),
+                                                  lane_prev(ac[r]), past[j], s_coef, ncol, air,
+                                                  lam2));
+        q2[j] = (upd1 && ((ru >> r) & 1u) && col_upd) ? v : (C)0;
+      }
+    } else if (air1 ? !air_uniform : any_nonair(bc)) {
       level1(std::true_type());
-    else
+    } else {
       level1(std::false_type());
+    }
 
     // ---- receivers (before injection) and sources on P^{n+1}[k]
     if (nrec != 0) {
@@ -487,10 +516,22 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>()))
           if (((rs >> o) & 1u) && owned_lane) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
         }
       };
-      if (any_nonair(bp))
+      if constexpr (SCHEME == 2) {
+#pragma unroll
+        for (int o = 0; o < kFTY; ++o) {
+          const int j = o + 1;
+          const unsigned pmv = pm_of(bp, j);
+          C v = centred_node<C>(pmv & 0xFFu, pmv >> 8, q1[j], q2[j], q0[j], q1[j + 1], q1[j - 1],
+                                lane_next(q1[j]), lane_prev(q1[j]), am[o + 2], s_coef, ncol, air,
+                                lam2);
+          v = (((ru >> (o + 2)) & 1u) && col_upd) ? v : (C)0;
+          if (((rs >> o) & 1u) && owned_lane) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
+        }
+      } else if (any_nonair(bp)) {
         level2(std::true_type());
-      else
+      } else {
         level2(std::false_type());
+      }
     }
   };
 
@@ -532,12 +573,12 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S>()))
 
 // Air-row plane: bit b of word (z * ntx + t) * nyw + w is set when every voxel of row
 // y = 32 w + b - 1 of slice z in columns [60 t - 1, 60 t + 61) -- the columns whose first-level
-// results wave tile t uses -- is plain air (packed value 0x0086: position 0x86, material 0);
-// rows outside the grid are set. One wave per word, lanes over the columns; blocks stride over
-// words and slices.
+// results wave tile t uses -- is plain air (packed value `air`: position 0x86 for the forward
+// scheme, 0x80 for the centred one; material 0); rows outside the grid are set. One wave per
+// word, lanes over the columns; blocks stride over words and slices.
 __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict__ pm,
                                                       uint32_t *__restrict__ out, int X, int Y,
-                                                      int Z, int ntx, int nyw) {
+                                                      int Z, int ntx, int nyw, unsigned air) {
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
   const int x = t * kFO - 1 + lane;
@@ -548,7 +589,7 @@ __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict
       uint32_t word = 0;
       for (int b = 0; b < 32; ++b) {
         const int y = 32 * w + b - 1;
-        const bool other = col && y >= 0 && y < Y && slice[(size_t)y * X + x] != 0x0086u;
+        const bool other = col && y >= 0 && y < Y && slice[(size_t)y * X + x] != air;
         if (__ballot(other) == 0) word |= 1u << b;
       }
       if (lane == 0) out[((size_t)z * ntx + t) * nyw + w] = word;
@@ -560,17 +601,18 @@ size_t air_rows_words(int X, int Y, int Z) {
   return (size_t)((X + kFO - 1) / kFO) * (size_t)(Y / 32 + 2) * (size_t)Z;
 }
 
-hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z,
+hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z, int scheme,
                            hipStream_t stream) {
   const int ntx = (X + kFO - 1) / kFO, nyw = Y / 32 + 2;
   const dim3 grid((unsigned)ntx, (unsigned)std::min(nyw, 4096), (unsigned)std::min(Z, 4096));
-  hipLaunchKernelGGL(air_rows_kernel, grid, dim3(64), 0, stream, pm, rows, X, Y, Z, ntx, nyw);
+  const unsigned air = scheme == 2 ? 0x0080u : 0x0086u;
+  hipLaunchKernelGGL(air_rows_kernel, grid, dim3(64), 0, stream, pm, rows, X, Y, Z, ntx, nyw, air);
   return hipGetLastError();
 }
 
-FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int shape) {
-  const int kFTY = precision == 0 ? FusedTile<float>::TY
-                                  : (precision == 1 ? FusedTile<double>::TY : FusedTile<h16>::TY);
+FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int shape,
+                     int scheme) {
+  const int kFTY = fused_ty_host(precision, scheme);
   FusedGeom g;
   g.X = X;
   g.Y = Y;
@@ -579,7 +621,8 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   const int ntx = (X + kFO - 1) / kFO;
   // automatic: wide blocks for FP32 rooms at least 16 x tiles (960 columns) wide, where at
   // most 3 of 16 tiles fall past the grid
-  g.wide = !pool && precision == 0 &&
+  g.scheme = scheme;
+  g.wide = !pool && precision == 0 && scheme == 0 &&
            (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16));
   const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
   const int WY = g.wide ? FusedShape<true>::WY : FusedShape<false>::WY;
@@ -602,13 +645,20 @@ static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a
                                hipStream_t stream) {
   const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
   const Fused2Ctx<S, CT, C> p{a, g.X, g.Y, g.Z, g.zc};
+  if (g.scheme == 2) {
+    hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false, 2>), g.grid, g.block, lds,
+                       stream, p);
+    return hipGetLastError();
+  }
   if constexpr (std::is_same<S, float>::value) {
     if (g.wide) {
-      hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, true>), g.grid, g.block, lds, stream, p);
+      hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, true, 0>), g.grid, g.block, lds,
+                         stream, p);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false>), g.grid, g.block, lds, stream, p);
+  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false, 0>), g.grid, g.block, lds, stream,
+                     p);
   return hipGetLastError();
 }
 
@@ -620,15 +670,15 @@ hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
   const dim3 grid((unsigned)tiles);
   switch (precision) {
   case PFDTD_F32:
-    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true, false>), grid,
+    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true, false, 0>), grid,
                        dim3(64 * fused_waves<false>()), lds, stream, p);
     break;
   case PFDTD_F16:
-    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true, false>), grid,
+    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true, false, 0>), grid,
                        dim3(64 * fused_waves<false>()), lds, stream, p);
     break;
   case PFDTD_BF16:
-    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true, false>), grid,
+    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true, false, 0>), grid,
                        dim3(64 * fused_waves<false>()), lds, stream, p);
     break;
   default:

@@ -187,18 +187,19 @@ struct Fused2Args {
 struct FusedGeom {
   int X, Y, Z;
   int zc;
-  bool wide;  // 4 x 4-wave blocks (FP32), else 1 x 4
+  bool wide;   // 4 x 4-wave blocks (FP32, forward scheme), else 1 x 4
+  int scheme;  // 0 forward boundaries, 2 centred
   dim3 grid, block;
 };
 // Grid of the two-level sweep; `pool`: serving-pool launches (1 x 4 blocks); `shape`: one of
-// PFDTD_SWEEP_AUTO / _COLUMN / _WIDE.
+// PFDTD_SWEEP_AUTO / _COLUMN / _WIDE; `scheme`: 0 forward, 2 centred boundaries.
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false,
-                     int shape = 0);
+                     int shape = 0, int scheme = 0);
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.
 size_t air_rows_words(int X, int Y, int Z);
-hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z,
+hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z, int scheme,
                            hipStream_t stream);
 hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
                               hipStream_t stream);

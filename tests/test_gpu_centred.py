@@ -20,7 +20,12 @@ def centred_room(seed, dims=(21, 17, 13), n_src=3, n_rec=4, n_steps=70):
     neighbour on some axis are filled)."""
     sc = rooms.scene_random_small(seed, dims=dims, n_mat=3, n_boxes=3, n_src=n_src, n_rec=n_rec,
                                   n_steps=n_steps)
-    lab = sc.labels.copy()
+    sc.labels = fill_centred(sc.labels.copy())
+    return sc
+
+
+def fill_centred(lab):
+    """Fill the inside voxels that have no air neighbour on some axis until none is left."""
     for _ in range(20):
         pos, _, counts = oracle.classify_kow(lab)
         if counts[2] == 0:
@@ -37,8 +42,7 @@ def centred_room(seed, dims=(21, 17, 13), n_src=3, n_rec=4, n_steps=70):
             hi[axis] = slice(2, None)
             bad |= inside & ~pad[tuple(lo)] & ~pad[tuple(hi)]
         lab[bad] = 1
-    sc.labels = lab
-    return sc
+    return lab
 
 
 def run_gpu(sc, prec, scale=0, update_type=N.SRL, want_fields=False, step_mode=False):
@@ -167,3 +171,60 @@ def test_centred_rejects_degenerate_and_stale_domains(gpu):
         assert ei.value.code == N.E_STATE
         s.set_domain_labels(ok)
         s.run(5, N.F32)
+
+
+def centred_border_room(seed, dims=(185, 54, 22), n_steps=36):
+    """A room crossing several two-level wave tiles (60 columns, 6/8-row tiles) with single solid
+    voxels next to the tile borders, sources spread over it; filled for centred stencils."""
+    sc = rooms.scene_random_small(seed, dims=dims, n_mat=3, n_boxes=2, n_src=1, n_rec=1,
+                                  n_steps=n_steps)
+    X, Y, Z = dims
+    xs = [x for t in range(1, (X + 59) // 60) for x in (60 * t - 2, 60 * t + 1) if 2 < x < X - 3]
+    ys = [y for ty in (4, 6, 8) for m in range(ty, Y, 2 * ty) for y in (m - 2, m + 1) if 2 < y < Y - 3]
+    for k, (x, y) in enumerate((x, y) for x in xs for y in ys):
+        sc.labels[2 + (5 * k) % (Z - 4), y, x] = 2 + k % 2
+    sc.labels = fill_centred(sc.labels)
+    rng = np.random.default_rng(seed)
+    n = 24
+    sc.src_xyz = np.stack([rng.integers(2, X - 2, n), rng.integers(2, Y - 2, n),
+                           rng.integers(2, Z - 2, n)], axis=1).astype(np.int32)
+    sc.src_kind = np.ones(n, np.int32)
+    sc.signals = rng.uniform(-1, 1, (n, n_steps))
+    sc.rec_xyz = sc.src_xyz[::2].copy()
+    return sc
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+@pytest.mark.parametrize("prec", [N.F32, N.F64, N.F16, N.BF16])
+def test_centred_time_levels_bit_exact(gpu, prec, levels):
+    """The centred update through the two-level sweep (default) and single steps, on a room
+    several wave tiles wide with boundary voxels next to the tile borders."""
+    sc = centred_border_room(105)
+    pname, storage, scale, _ = PREC[prec]
+    pos, mat, _ = oracle.classify_kow(sc.labels)
+    sig = sc.signals if pname == "f64" else sc.signals.astype(np.float32)
+    ref, P_ref, Pp_ref = oracle.run(pos, mat, sc.material_table(), sc.src_xyz, sc.src_kind, sig,
+                                    sc.rec_xyz, sc.n_steps, precision=pname, storage=storage,
+                                    scale_exp=scale, literal=oracle.SCHEME_KOWALCZYK, threads=8,
+                                    want_fields=True)
+    raw = {N.F32: np.float32, N.F64: np.float64, N.F16: np.float16, N.BF16: np.uint16}[prec]
+    with Solver() as s:
+        s.set_time_levels(levels, 5)
+        s.set_update_type(N.SRL)
+        s.set_domain_labels(sc.labels)
+        s.set_materials(sc.material_table())
+        s.set_sources(sc.src_xyz, sc.src_kind, signal_f64=sc.signals,
+                      signal_f32=sc.signals.astype(np.float32))
+        s.set_receivers(sc.rec_xyz)
+        if scale:
+            s.prepare_fields(prec, scale)
+        s.run(sc.n_steps, prec)
+        got = s.responses(np.float64 if prec == N.F64 else np.float32)
+        P, Pp = s.fields_raw(raw)
+    if raw == np.float16:
+        P, Pp = P.astype(np.float32), Pp.astype(np.float32)
+    elif raw == np.uint16:
+        P, Pp = [(h.astype(np.uint32) << 16).view(np.float32) for h in (P, Pp)]
+    assert np.abs(ref).max() > 1e-3
+    assert np.array_equal(got, ref), f"max diff {np.abs(got - ref).max()}"
+    assert np.array_equal(P, P_ref) and np.array_equal(Pp, Pp_ref)
