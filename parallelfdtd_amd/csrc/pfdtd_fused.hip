@@ -25,6 +25,8 @@
 // The same kernel serves two callers: a context's domain (POOL = false) and the two-level jobs
 // of one serving-pool launch (POOL = true: the block finds its job in the launch table and takes
 // its arguments from the job descriptor; a job with one step left runs only the first level).
+// SCHEME selects the boundary formulation of a context's domain: forward (Bilbao, update types
+// SRL_FORWARD and SHARED) or centred (Kowalczyk, update type SRL).
 //
 // Source injection of step n+1 and the receiver gather of step n happen inside the kernel on
 // P^{n+1}, in the reference's order (gather before the next injection; sources in index order),
