@@ -195,10 +195,10 @@ def test_two_level_domain_reset_with_same_voxel_count(gpu):
 
 
 @pytest.mark.parametrize("shape", [N.SWEEP_COLUMN, N.SWEEP_WIDE])
-@pytest.mark.parametrize("prec", [N.F32, N.F16])
+@pytest.mark.parametrize("prec", [N.F32, N.F64, N.F16])
 def test_two_level_block_shapes(gpu, shape, prec):
-    """Both block shapes on a room whose 4 x tiles leave wide blocks partly past the grid (the
-    16-bit sweep always uses columns), with boundary voxels on the tile borders."""
+    """Both block shapes on a room whose 4 x tiles leave wide blocks partly past the grid (FP64
+    and the 16-bit sweep always use columns), with boundary voxels on the tile borders."""
     sc = border_voxel_scene(91, dims=(250, 54, 20), n_steps=30)
     assert_same(*run_both(sc, prec, zchunk=7, shape=shape))
 
