@@ -62,9 +62,10 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
     def record(B, levels, ry, zc, ms, shape=None):
         per_src = nodes * B / (ms * 1e-3) / 1e9
         gbs = nodes * (BYTES_F32["per_voxel"] + BYTES_F32["per_slot"] * B) / (ms * 1e-3) / 1e9
+        # six significant digits: the speedups below are ratios of these values
         cand = {"B": B, "levels": levels, "rows": ry, "zchunk": zc, "shape": shape,
-                "ms_per_step": round(ms, 4), "per_source_gvox": round(per_src, 2),
-                "alg_GBps": round(gbs, 1)}
+                "ms_per_step": float(f"{ms:.6g}"), "per_source_gvox": float(f"{per_src:.6g}"),
+                "alg_GBps": float(f"{gbs:.6g}")}
         out["candidates"].append(cand)
         log(json.dumps(cand))
         return cand
@@ -98,7 +99,7 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
         base = table.get("1")
         if base:
             for v in table.values():
-                v["per_source_speedup_vs_B1"] = round(v["per_source_gvox"] / base["per_source_gvox"], 3)
+                v["per_source_speedup_vs_B1"] = float(f"{v['per_source_gvox'] / base['per_source_gvox']:.4g}")
     top = max(out["best"].values(), key=lambda c: c["per_source_gvox"])
     out["choice"] = {k: top[k] for k in ("B", "levels", "rows", "zchunk", "shape")}
     return out
