@@ -229,9 +229,10 @@ int pfdtd_set_kernel_timing(pfdtd_ctx *ctx, int enabled);
 int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
 /* Time levels advanced per sweep of the field (1 or 2; 0 = default = 2, the faster path on
  * MI355X in every precision) and slices per z-chunk of the two-level update (0 = default =
- * automatic: the longest of 32, 16 and 8 slices that gives the sweep 4096 blocks). Batched
- * contexts always advance one level per sweep; both update formulations (forward and centred
- * boundaries) take the two-level sweep. With 2, pfdtd_run advances pairs of steps in one kernel
+ * automatic: the longest of 32 (16 for 16-bit storage), 16 and 8 slices that gives the sweep
+ * 4096 blocks). Batched contexts always advance one level per sweep; both update formulations
+ * (forward and centred boundaries) take the two-level sweep. With 2, pfdtd_run advances pairs
+ * of steps in one kernel
  * that reads P^n, P^{n-1} and the byte planes once and writes P^{n+1}, P^{n+2} (four field buffers;
  * sources of step n+1 injected and receivers of step n gathered inside the kernel), falling back
  * to single steps for an odd remainder and whenever a source sits on slice z = 0 or z = Z-1.
