@@ -227,3 +227,21 @@ def test_two_level_automatic_zchunk(gpu, prec):
     with Solver() as s:
         with pytest.raises(N.PfdtdError):
             s.set_time_levels(2, -1)
+
+
+@pytest.mark.parametrize("dims", [(3, 3, 3), (4, 5, 3), (61, 3, 4), (3, 70, 5), (121, 33, 6),
+                                  (241, 7, 9)])
+@pytest.mark.parametrize("prec", [N.F32, N.F16])
+def test_two_level_degenerate_shapes(gpu, dims, prec):
+    """Rooms down to one interior voxel, widths of one tile plus a column, heights below one
+    wave tile, and (241 columns) a wide block with three of its four tiles past the grid."""
+    X, Y, Z = dims
+    sc = rooms.scene_random_small(94, dims=dims, n_mat=2, n_boxes=0, n_src=1, n_rec=1,
+                                  n_steps=13)
+    sc.src_xyz = np.array([[X // 2, Y // 2, Z // 2]], np.int32)
+    sc.src_kind = np.array([0], np.int32)
+    sc.rec_xyz = np.array([[X // 2, Y // 2, Z // 2], [1, 1, 1], [X - 2, Y - 2, Z - 2]], np.int32)
+    for shape in (N.SWEEP_COLUMN, N.SWEEP_WIDE):
+        got, ref, F, F_ref = run_both(sc, prec, zchunk=0, shape=shape)
+        assert np.abs(ref).max() > 0
+        assert_same(got, ref, F, F_ref)
