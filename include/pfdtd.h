@@ -201,6 +201,13 @@ int pfdtd_run(pfdtd_ctx *ctx, int precision, int n_steps, pfdtd_interrupt_cb int
  * pfdtd_prepare_fields and a response length (n_steps) set by pfdtd_begin_steps. */
 int pfdtd_begin_steps(pfdtd_ctx *ctx, int precision, int n_steps);
 int pfdtd_step(pfdtd_ctx *ctx, int step, int direction);
+/* Steps first_step .. first_step + n_steps - 1 forward in one call: the same fields and
+ * responses as that many pfdtd_step(ctx, s, +1) calls, advanced in pairs by the two-level sweep
+ * where pfdtd_run would use it (one synchronisation at the end). first_step + n_steps may not
+ * exceed the response length set by pfdtd_begin_steps. No reference counterpart: the
+ * reference's capture loop (App::runCapture, src/App.cpp:350-379) launches one step at a time;
+ * FDTD::App::runCapture calls this for the steps between two captures. */
+int pfdtd_steps(pfdtd_ctx *ctx, int first_step, int n_steps);
 
 /* Responses [n_rec][n_steps] of the last run or step sequence (kernels3d.cu:184-191). */
 int pfdtd_get_responses_f32(pfdtd_ctx *ctx, float *out, int64_t capacity);

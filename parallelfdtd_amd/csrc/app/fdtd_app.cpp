@@ -577,13 +577,36 @@ void App::runCapture() {
   responses_.assign((size_t)steps * m_parameters.getNumReceivers(), 0.f);
   check(pfdtd_begin_steps(ctx_, PFDTD_F32, (int)steps), "runCapture");
   current_step_ = 0;
-  float total = 0.f;
-  for (unsigned int i = 0; i < steps; ++i) {
-    executeStep();
-    total += time_per_step_;
-    if (m_interrupt && m_interrupt()) break;
+  if (step_direction_ == 1) {
+    // the steps between two scheduled captures advance in one call (the two-level sweep where
+    // it applies), at most kCaptureSegment steps at a time so the interrupt is still polled;
+    // results equal the reference's step-by-step loop
+    constexpr unsigned int kCaptureSegment = 100;
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned int done = 0;
+    while (done < steps) {
+      unsigned int stop = std::min(steps, done + kCaptureSegment);
+      for (unsigned int s : step_to_capture_)
+        if (s > done && s < stop) stop = s;
+      for (unsigned int s : mesh_to_capture_)
+        if (s > done && s < stop) stop = s;
+      check(pfdtd_steps(ctx_, (int)done, (int)(stop - done)), "runCapture");
+      done = stop;
+      current_step_ = done;
+      captureScheduled();
+      if (m_interrupt && m_interrupt()) break;
+    }
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    time_per_step_ = done ? (float)(sec / done) : 0.f;
+  } else {
+    float total = 0.f;
+    for (unsigned int i = 0; i < steps; ++i) {
+      executeStep();
+      total += time_per_step_;
+      if (m_interrupt && m_interrupt()) break;
+    }
+    time_per_step_ = steps ? total / (float)steps : 0.f;
   }
-  time_per_step_ = steps ? total / (float)steps : 0.f;
   if (!responses_.empty())
     check(pfdtd_get_responses_f32(ctx_, responses_.data(), (int64_t)responses_.size()), "runCapture");
 }
@@ -594,6 +617,11 @@ void App::executeStep() {
   const auto t0 = std::chrono::steady_clock::now();
   check(pfdtd_step(ctx_, (int)current_step_, step_direction_), "executeStep");
   current_step_ += step_direction_;
+  captureScheduled();
+  time_per_step_ = (float)std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void App::captureScheduled() {
   const unsigned int X = (unsigned int)m_mesh.getDimX(), Y = (unsigned int)m_mesh.getDimY(),
                      Z = (unsigned int)m_mesh.getDimZ();
   for (size_t i = 0; i < step_to_capture_.size(); ++i) {
@@ -616,7 +644,6 @@ void App::executeStep() {
     check(pfdtd_capture_field(ctx_, mesh_captures_.back().data(), m_mesh.getNumberOfElements()),
           "captureMesh");
   }
-  time_per_step_ = (float)std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // resetPressureMesh, src/App.cpp:397-401

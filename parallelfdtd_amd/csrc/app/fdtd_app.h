@@ -339,6 +339,7 @@ class App {
   void ensureContext();
   void initializeMesh();
   void uploadSourcesAndReceivers(bool single_precision_only);
+  void captureScheduled();  // the slice and mesh captures scheduled at current_step_
   void check(int rc, const char *what);
 
   pfdtd_ctx *ctx_ = nullptr;

@@ -1010,6 +1010,34 @@ int pfdtd_reset_fields(pfdtd_ctx *c) {
   return zero_fields(c);
 }
 
+// The two-level sweep reads position and material as one 16-bit plane (2 B per voxel), built
+// once per domain with its air-row bits; without room for the plane *fused is cleared and the
+// caller takes single steps.
+static int ensure_pm(pfdtd_ctx *c, bool *fused) {
+  if (!*fused || c->pm_ok) return PFDTD_OK;
+  if (!c->d_pm && hipMalloc((void **)&c->d_pm, sizeof(uint16_t) * (size_t)c->N) != hipSuccess) {
+    (void)hipGetLastError();
+    c->d_pm = nullptr;
+    *fused = false;
+    return PFDTD_OK;
+  }
+  HIP_TRY(launch_pack_pm(c->d_pos, c->d_mat, c->d_pm, c->N, c->stream));
+  // the air-row bits only save loads: without room for them the sweep reads every row
+  const size_t words = air_rows_words((int)c->X, (int)c->Y, (int)c->Z);
+  if (!c->d_air_rows && hipMalloc((void **)&c->d_air_rows, sizeof(uint32_t) * words) != hipSuccess) {
+    (void)hipGetLastError();
+    c->d_air_rows = nullptr;
+  }
+  if (c->d_air_rows &&
+      launch_air_rows(c->d_pm, c->d_air_rows, (int)c->X, (int)c->Y, (int)c->Z, c->scheme,
+                      c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    dfree(c->d_air_rows);
+  }
+  c->pm_ok = true;
+  return PFDTD_OK;
+}
+
 int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb interrupt,
               pfdtd_progress_cb progress, void *user, int *steps_done, double *ms_elapsed) {
   int rc = check_ctx(c);
@@ -1039,30 +1067,8 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   if (n_steps > 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, 0), c->stream));
 
   bool fused = fused_eligible(c);
-  if (fused && !c->pm_ok) {
-    // the two-level sweep reads position and material as one 16-bit plane (2 B per voxel);
-    // without room for it the run takes single steps
-    if (!c->d_pm && hipMalloc((void **)&c->d_pm, sizeof(uint16_t) * (size_t)c->N) != hipSuccess) {
-      (void)hipGetLastError();
-      c->d_pm = nullptr;
-      fused = false;
-    } else {
-      HIP_TRY(launch_pack_pm(c->d_pos, c->d_mat, c->d_pm, c->N, c->stream));
-      // the air-row bits only save loads: without room for them the sweep reads every row
-      const size_t words = air_rows_words((int)c->X, (int)c->Y, (int)c->Z);
-      if (!c->d_air_rows && hipMalloc((void **)&c->d_air_rows, sizeof(uint32_t) * words) != hipSuccess) {
-        (void)hipGetLastError();
-        c->d_air_rows = nullptr;
-      }
-      if (c->d_air_rows &&
-          launch_air_rows(c->d_pm, c->d_air_rows, (int)c->X, (int)c->Y, (int)c->Z, c->scheme,
-                          c->stream) != hipSuccess) {
-        (void)hipGetLastError();
-        dfree(c->d_air_rows);
-      }
-      c->pm_ok = true;
-    }
-  }
+  rc = ensure_pm(c, &fused);
+  if (rc) return rc;
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
                                   c->fshape, c->scheme);
   int done = 0;
@@ -1158,6 +1164,56 @@ int pfdtd_step(pfdtd_ctx *c, int step, int direction) {
   c->last_direction = direction;
   if (step >= 0 && step < c->resp_steps)
     HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, step, -1), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return PFDTD_OK;
+}
+
+int pfdtd_steps(pfdtd_ctx *c, int first_step, int n_steps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->d_P || c->precision < 0) return fail(PFDTD_E_STATE, "fields not prepared");
+  if (first_step < 0 || n_steps < 0 || (int64_t)first_step + n_steps > c->resp_steps)
+    return fail(PFDTD_E_ARG, "steps %d..%d outside the %d steps of pfdtd_begin_steps", first_step,
+                first_step + n_steps - 1, c->resp_steps);
+  rc = check_runnable(c);
+  if (rc) return rc;
+  if (n_steps == 0) return PFDTD_OK;
+  // after a backward step the next forward one keeps the pointers (the flip rule of
+  // pfdtd_step): take it on its own
+  if (c->last_direction != 1) {
+    rc = pfdtd_step(c, first_step, 1);
+    if (rc) return rc;
+    ++first_step;
+    --n_steps;
+    if (n_steps == 0) return PFDTD_OK;
+  }
+  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc, c->lgB);
+  bool fused = fused_eligible(c);
+  rc = ensure_pm(c, &fused);
+  if (rc) return rc;
+  const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
+                                  c->fshape, c->scheme);
+  // the sequence of pfdtd_step calls: inject(s), update, gather(s) for s = first .. first+n-1,
+  // with the injection of each following step applied right after the previous gather
+  HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, first_step), c->stream));
+  for (int k = 0; k < n_steps;) {
+    const int step = first_step + k;
+    if (fused && k + 2 <= n_steps) {
+      HIP_TRY(launch_update2_any(c, fg, step));
+      const int next = k + 2 < n_steps ? step + 2 : -1;
+      HIP_TRY(launch_inject_gather(c->precision, c->d_P2, inject_args(c, step + 1, next), c->stream));
+      std::swap(c->d_P, c->d_P2);
+      std::swap(c->d_Pp, c->d_Pp2);
+      k += 2;
+    } else {
+      HIP_TRY(launch_update_any(c, g));
+      const int next = k + 1 < n_steps ? step + 1 : -1;
+      HIP_TRY(launch_inject_gather(c->precision, c->d_Pp, inject_args(c, step, next), c->stream));
+      std::swap(c->d_P, c->d_Pp);
+      k += 1;
+    }
+  }
+  c->last_direction = 1;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return PFDTD_OK;
 }

@@ -66,6 +66,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_run": [_p, C.c_int, C.c_int, INTERRUPT_CB, PROGRESS_CB, _p, C.POINTER(C.c_int), _pdbl],
     "pfdtd_begin_steps": [_p, C.c_int, C.c_int],
     "pfdtd_step": [_p, C.c_int, C.c_int],
+    "pfdtd_steps": [_p, C.c_int, C.c_int],
     "pfdtd_get_responses_f32": [_p, _p, _i64],
     "pfdtd_get_responses_f64": [_p, _p, _i64],
     "pfdtd_get_response_shape": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],

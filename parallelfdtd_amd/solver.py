@@ -210,6 +210,10 @@ class Solver:
     def step(self, step: int, direction: int = 1) -> None:
         self._c("pfdtd_step", step, direction)
 
+    def steps(self, first_step: int, n_steps: int) -> None:
+        """n_steps forward steps from first_step: the same as that many step() calls."""
+        self._c("pfdtd_steps", first_step, n_steps)
+
     def response_shape(self) -> tuple[int, int]:
         r, s = C.c_int(), C.c_int()
         self._c("pfdtd_get_response_shape", C.byref(r), C.byref(s))

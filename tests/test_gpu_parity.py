@@ -226,3 +226,54 @@ def test_tiling_invariance(gpu, rows, zchunk, prec):
             P, Pp = s.fields_raw(np.float64 if pname == "f64" else np.float32)
             assert np.array_equal(P, P_ref) and np.array_equal(Pp, Pp_ref)
     assert np.array_equal(got, ref)
+
+
+def _step_sequence(sc, prec, plan, scale=0):
+    """Run `plan` (a list of ("step", n, direction) / ("steps", first, count)) after
+    begin_steps; return responses and raw fields."""
+    raw = np.float32 if prec == N.F32 else np.float16
+    with Solver() as s:
+        s.load_scene(sc)
+        if scale:
+            s.prepare_fields(prec, scale)
+        s.begin_steps(sc.n_steps, prec)
+        for op in plan:
+            if op[0] == "step":
+                s.step(op[1], op[2])
+            else:
+                s.steps(op[1], op[2])
+        return s.responses(np.float32), s.fields_raw(raw)
+
+
+@pytest.mark.parametrize("prec,scale", [(N.F32, 0), (N.F16, 9)])
+def test_steps_equal_single_step_calls(gpu, prec, scale):
+    """pfdtd_steps over segments of odd and even lengths (two-level sweep inside) gives the
+    fields and responses of the same steps taken one pfdtd_step call at a time, including after
+    a backward step (the flip rule)."""
+    sc = rooms.scene_random_small(52, dims=(130, 40, 21), n_mat=3, n_boxes=4, n_src=3, n_rec=5,
+                                  n_steps=60)
+    one = [("step", n, 1) for n in range(25)] + [("step", 25, -1)] + \
+          [("step", n, 1) for n in range(26, 60)]
+    seg = [("steps", 0, 7), ("steps", 7, 2), ("steps", 9, 1), ("steps", 10, 15), ("step", 25, -1),
+           ("steps", 26, 13), ("steps", 39, 21)]
+    r1, f1 = _step_sequence(sc, prec, one, scale)
+    r2, f2 = _step_sequence(sc, prec, seg, scale)
+    assert np.abs(r1).max() > 0
+    assert np.array_equal(r1, r2)
+    assert np.array_equal(f1[0], f2[0]) and np.array_equal(f1[1], f2[1])
+
+
+def test_steps_match_run_and_reject_overrun(gpu):
+    sc = rooms.scene_random_small(53, dims=(70, 30, 17), n_steps=41)
+    with Solver() as s:
+        s.load_scene(sc)
+        s.run(sc.n_steps, N.F32)
+        ref = s.responses(np.float32)
+        s.begin_steps(sc.n_steps, N.F32)
+        s.steps(0, sc.n_steps)
+        assert np.array_equal(s.responses(np.float32), ref)
+        s.begin_steps(sc.n_steps, N.F32)
+        with pytest.raises(N.PfdtdError):
+            s.steps(30, 12)
+        with pytest.raises(N.PfdtdError):
+            s.steps(-1, 2)
