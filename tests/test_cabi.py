@@ -94,3 +94,22 @@ def test_voxel_grid_without_device():
     assert voxel_grid(b.vertices, 0.1, (32, 4, 1)) == (32, 16, 13)
     with pytest.raises(native.PfdtdError):
         voxel_grid(b.vertices, 0.0)
+
+
+def header_defines():
+    return {m.group(1): int(m.group(2)) for m in
+            re.finditer(r"#define\s+(PFDTD_[A-Z0-9_]+)\s+\(?(-?\d+)\)?", HEADER.read_text())}
+
+
+def test_binding_constants_match_header():
+    """The Python binding's numeric constants are the header's."""
+    d = header_defines()
+    pairs = {"PFDTD_F32": native.F32, "PFDTD_F64": native.F64, "PFDTD_F16": native.F16,
+             "PFDTD_BF16": native.BF16, "PFDTD_SWEEP_AUTO": native.SWEEP_AUTO,
+             "PFDTD_SWEEP_COLUMN": native.SWEEP_COLUMN, "PFDTD_SWEEP_WIDE": native.SWEEP_WIDE,
+             "PFDTD_OK": native.OK, "PFDTD_E_ARG": native.E_ARG, "PFDTD_E_HIP": native.E_HIP,
+             "PFDTD_E_NOMEM": native.E_NOMEM, "PFDTD_E_STATE": native.E_STATE,
+             "PFDTD_E_DOMAIN": native.E_DOMAIN, "PFDTD_SRC_HARD": native.SRC_HARD,
+             "PFDTD_SRC_SOFT": native.SRC_SOFT, "PFDTD_SRC_TRANSPARENT": native.SRC_TRANSPARENT}
+    for name, value in pairs.items():
+        assert d[name] == value, name
