@@ -64,6 +64,11 @@ def lib() -> C.CDLL:
                                 _p, _p, _p]
         L.orc_capture_slice_f.restype = C.c_int
         L.orc_capture_slice_f.argtypes = [_p, _p, _i64, _i64, _i64, _i64, C.c_int, _p, _p]
+        L.orc_step_calls_f.restype = C.c_int
+        L.orc_step_calls_f.argtypes = [_p, _p, _i64, _i64, _i64, _p, C.c_int, C.c_uint,
+                                       C.c_double, _p, _p, _p, C.c_int, _p, C.c_int, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, _p, _p, C.c_int, _p,
+                                       _p, _p]
         L.orc_f32_to_f16.restype = C.c_uint16
         L.orc_f32_to_f16.argtypes = [C.c_float]
         L.orc_f16_to_f32.restype = C.c_float
@@ -180,3 +185,59 @@ def run(pos, mat, table, src_xyz, src_kind, signals, rec_xyz, n_steps, *, precis
     if rc != 0:
         raise ValueError(f"oracle run failed with {rc}")
     return (resp, P, Pp) if want_fields else resp
+
+
+def _table20(table, dt):
+    t = np.ascontiguousarray(table, np.float32)
+    if t.ndim == 1:
+        t = t[:, None]
+    tab = np.zeros((t.shape[0], 20), np.float32)
+    tab[:, : t.shape[1]] = t
+    return np.ascontiguousarray(tab.astype(dt))
+
+
+def step_calls(pos, mat, table, src_xyz, src_kind, signals, rec_xyz, n_steps, steps, dirs, *,
+               octave=0, lam=math.sqrt(1.0 / 3.0), literal=0, storage=STORE_F32, scale_exp=0,
+               threads=1):
+    """Single-step mode (launchFDTD3dStep per call, FP32): call c advances with step index
+    steps[c] and direction dirs[c]. Returns (responses [n_rec][n_steps], current, past)."""
+    pos = np.ascontiguousarray(pos, np.uint8)
+    mat = np.ascontiguousarray(mat, np.uint8)
+    Z, Y, X = pos.shape
+    tab = _table20(table, np.float32)
+    src_xyz = np.ascontiguousarray(src_xyz, np.int32).reshape(-1, 3)
+    src_kind = np.ascontiguousarray(src_kind, np.int32).reshape(-1)
+    rec_xyz = np.ascontiguousarray(rec_xyz, np.int32).reshape(-1, 3)
+    n_src, n_rec = src_xyz.shape[0], rec_xyz.shape[0]
+    sig = np.zeros((n_src, n_steps), np.float32)
+    if n_src:
+        s = np.asarray(signals).reshape(n_src, -1)
+        k = min(n_steps, s.shape[1])
+        sig[:, :k] = s[:, :k].astype(np.float32)
+    st = np.ascontiguousarray(steps, np.int32)
+    dr = np.ascontiguousarray(dirs, np.int32)
+    assert st.shape == dr.shape
+    resp = np.zeros((n_rec, n_steps), np.float32)
+    P = np.zeros((Z, Y, X), np.float32)
+    Pp = np.zeros((Z, Y, X), np.float32)
+    rc = lib().orc_step_calls_f(_ptr(pos), _ptr(mat), X, Y, Z, _ptr(tab), tab.shape[0], octave,
+                                lam, _ptr(src_xyz), _ptr(src_kind), _ptr(sig), n_src,
+                                _ptr(rec_xyz), n_rec, n_steps, literal, storage, scale_exp,
+                                threads, _ptr(st), _ptr(dr), st.size, _ptr(resp), _ptr(P),
+                                _ptr(Pp))
+    if rc != 0:
+        raise ValueError(f"oracle step calls failed with {rc}")
+    return resp, P, Pp
+
+
+def capture_slice(P, pos, slice_, orientation):
+    """orc_capture_slice_f: (p, pos) planes of an FP32 field, or None when out of range."""
+    P = np.ascontiguousarray(P, np.float32)
+    pos = np.ascontiguousarray(pos, np.uint8)
+    Z, Y, X = P.shape
+    shape = {0: (Y, X), 1: (Z, X), 2: (Z, Y)}.get(orientation, (1, 1))
+    op = np.full(shape, np.nan, np.float32)
+    oq = np.full(shape, 0xEE, np.uint8)
+    rc = lib().orc_capture_slice_f(_ptr(P), _ptr(pos), X, Y, Z, slice_, orientation, _ptr(op),
+                                   _ptr(oq))
+    return None if rc else (op, oq)

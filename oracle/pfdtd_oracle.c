@@ -709,7 +709,11 @@ int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
   return 0;
 }
 
-/* captureSliceFast / captureSliceKernel, visualizationUtils.cu:130-254, :500-527 */
+/* captureSliceFast / captureSliceKernel, visualizationUtils.cu:130-254, :500-527.
+ * xy (orientation 0) is a plain copy of slice z (getSlice / getPositionSlice, cudaMesh.h:601-646).
+ * xz and yz are gathered by captureSliceKernel into calloc'd planes with the limit
+ * dim * (partition size - 1) (visualizationUtils.cu:178, :190, :522): with the whole grid in one
+ * partition of Z slices, the rows z = 0..Z-2 are written and row z = Z-1 stays 0. */
 int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y, int64_t Z,
                         int64_t slice, int orientation, float *out_p, uint8_t *out_pos) {
   const int64_t XY = X * Y;
@@ -727,8 +731,9 @@ int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y
     if (slice >= Y) return -1;
     for (int64_t z = 0; z < Z; ++z)
       for (int64_t x = 0; x < X; ++x) {
-        out_p[z * X + x] = P[z * XY + slice * X + x];
-        out_pos[z * X + x] = pos[z * XY + slice * X + x];
+        const int keep = z < Z - 1;
+        out_p[z * X + x] = keep ? P[z * XY + slice * X + x] : 0.f;
+        out_pos[z * X + x] = keep ? pos[z * XY + slice * X + x] : 0;
       }
     return 0;
   }
@@ -736,12 +741,87 @@ int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y
     if (slice >= X) return -1;
     for (int64_t z = 0; z < Z; ++z)
       for (int64_t y = 0; y < Y; ++y) {
-        out_p[z * Y + y] = P[z * XY + y * X + slice];
-        out_pos[z * Y + y] = pos[z * XY + y * X + slice];
+        const int keep = z < Z - 1;
+        out_p[z * Y + y] = keep ? P[z * XY + y * X + slice] : 0.f;
+        out_pos[z * Y + y] = keep ? pos[z * XY + y * X + slice] : 0;
       }
     return 0;
   }
   return -1;
+}
+
+/* launchFDTD3dStep called once per entry of steps[] / dirs[] (kernels3d.cu:376-482), as
+ * App::executeStep drives it (App.cpp:407-436). */
+int orc_step_calls_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+                     const float *mat_table, int n_mat, unsigned octave, double lambda,
+                     const int32_t *src_xyz, const int32_t *src_type, const float *signals,
+                     int n_src, const int32_t *rec_xyz, int n_rec, int n_steps, int literal,
+                     int storage, int scale_exp, int threads, const int32_t *steps,
+                     const int32_t *dirs, int n_calls, float *responses, float *P_out,
+                     float *Pp_out) {
+  int rc = check_common(pos, mat, X, Y, Z, n_mat, octave, literal, n_src, n_rec, n_steps);
+  if (rc) return rc;
+  if (!mat_table || storage < 0 || storage > 2 || scale_exp < -100 || scale_exp > 100) return -1;
+  if (n_calls < 0 || (n_calls > 0 && (!steps || !dirs))) return -1;
+  for (int c = 0; c < n_calls; ++c)
+    if (steps[c] < 0 || steps[c] >= n_steps || (dirs[c] != 1 && dirs[c] != -1)) return -1;
+  if (threads < 1) threads = 1;
+  float params[4];
+  orc_param_vector_f(lambda, octave, params);
+  float beta_of_mat[256];
+  for (int m = 0; m < 256; ++m) {
+    const int64_t slot = orc_material_slot((unsigned)m, octave, (literal & ORC_SCHEME_KOWALCZYK) ? 0 : (literal & 1));
+    beta_of_mat[m] = slot < (int64_t)n_mat * ORC_COEF_NUM ? mat_table[slot] : 0.f;
+  }
+  const int64_t N = X * Y * Z;
+  float *A = (float *)calloc((size_t)N, sizeof(float));
+  float *B = (float *)calloc((size_t)N, sizeof(float));
+  if (!A || !B) {
+    free(A);
+    free(B);
+    return -1;
+  }
+  float *P = A, *Pp = B;
+  const float scale = ldexpf(1.f, scale_exp);
+  const float inv_scale = ldexpf(1.f, -scale_exp);
+  if (responses) memset(responses, 0, sizeof(float) * (size_t)n_rec * (size_t)n_steps);
+  int past_direction = 1; /* the function-static past_step_directon, kernels3d.cu:386 */
+  for (int c = 0; c < n_calls; ++c) {
+    const int n = steps[c];
+    /* source loop: samples of step `n` into the current field, kernels3d.cu:400-409 */
+    for (int s = 0; s < n_src; ++s) {
+      const int32_t *q = src_xyz + 3 * s;
+      if (!in_grid(q, X, Y, Z)) continue;
+      const int64_t i = lin(q, X, Y);
+      const float smp = signals[(size_t)s * (size_t)n_steps + (size_t)n] * scale;
+      if (src_type[s] == ORC_SRC_HARD || (literal & 2))
+        P[i] = store_round(smp, storage);
+      else
+        P[i] = store_round(P[i] + smp, storage);
+    }
+    if (literal & ORC_SCHEME_KOWALCZYK)
+      sweep_kow_f(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, storage, threads);
+    else
+      sweep_f(pos, mat, P, Pp, beta_of_mat, params[0], params[1], X, Y, Z, storage, threads);
+    /* flip only while the direction is unchanged, kernels3d.cu:462-465 */
+    if (past_direction == dirs[c]) {
+      float *t = P;
+      P = Pp;
+      Pp = t;
+    }
+    past_direction = dirs[c];
+    /* receiver loop: sample `n` from the current field, kernels3d.cu:470-475 */
+    for (int r = 0; r < n_rec; ++r) {
+      const int32_t *q = rec_xyz + 3 * r;
+      if (!in_grid(q, X, Y, Z) || !responses) continue;
+      responses[(size_t)r * (size_t)n_steps + (size_t)n] = P[lin(q, X, Y)] * inv_scale;
+    }
+  }
+  if (P_out) memcpy(P_out, P, sizeof(float) * (size_t)N);
+  if (Pp_out) memcpy(Pp_out, Pp, sizeof(float) * (size_t)N);
+  free(A);
+  free(B);
+  return 0;
 }
 
 /* ------------------------------------------------------------------------------------ */

@@ -154,10 +154,33 @@ int orc_run_d(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
 
 /* Slice gather of a field and the position bytes: captureSliceFast / captureSliceKernel,
  * src/kernels/visualizationUtils.cu:130-254, :500-527. orientation 0 = xy at z=slice
- * (out[y*X+x]), 1 = xz at y=slice (out[z*X+x]), 2 = yz at x=slice (out[z*Y+y]).
- * Returns -1 when the slice is out of range (the reference logs and skips). */
+ * (out[y*X+x]), 1 = xz at y=slice (out[z*X+x]), 2 = yz at x=slice (out[z*Y+y]); in xz and yz
+ * the row z = Z-1 is 0 (the kernel's limit dim * (partition size - 1), one partition).
+ * Returns -1 when the slice is out of range (the reference logs and skips).
+ * Pinned by numpy slicing of random fields in every orientation and the out-of-range rule
+ * (tests/test_oracle_capture.py). */
 int orc_capture_slice_f(const float *P, const uint8_t *pos, int64_t X, int64_t Y, int64_t Z,
                         int64_t slice, int orientation, float *out_p, uint8_t *out_pos);
+
+/* Single-step mode: launchFDTD3dStep (kernels3d.cu:376-482) called n_calls times as
+ * App::executeStep calls it (App.cpp:407-436), call c with step index steps[c] (0 <= steps[c] <
+ * n_steps) and direction dirs[c] (+1 or -1). Each call injects the source samples of step
+ * steps[c] (signals [n_src][n_steps]) into the current field, runs the update (current, past ->
+ * past buffer), flips the field pointers only when dirs[c] equals the previous call's
+ * direction (initially +1: the function-static past_step_directon, kernels3d.cu:386, :462-465),
+ * then writes response[r][steps[c]] from the current field. Responses no call writes are 0.
+ * Other arguments and the return code as orc_run_f.
+ * Pinned by (tests/test_oracle_steps.py): all-forward calls equal orc_run_f bit for bit; on a
+ * direction change the current field stays p^K and the past buffer holds p^{K+1}, both bit for
+ * bit; with rigid walls (FP64 arithmetic through FP32 fields aside) reversed calls retrace the
+ * forward fields p^{K-1}, p^{K-2}, ... to rounding. */
+int orc_step_calls_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+                     const float *mat_table, int n_mat, unsigned octave, double lambda,
+                     const int32_t *src_xyz, const int32_t *src_type, const float *signals,
+                     int n_src, const int32_t *rec_xyz, int n_rec, int n_steps, int literal,
+                     int storage, int scale_exp, int threads, const int32_t *steps,
+                     const int32_t *dirs, int n_calls, float *responses, float *P_out,
+                     float *Pp_out);
 
 /* Triangle-mesh solid voxelization into a label grid (0 = air inside the mesh, m+1 = solid
  * voxel of material m), the input of orc_classify_labels / orc_classify_labels_kow.

@@ -140,3 +140,112 @@ def test_absorbing_walls_lose_energy_relative_to_rigid():
         late = r[0, steps // 2:]
         e.append(float(np.sum((late - late.mean()) ** 2)))
     assert e[1] < 0.5 * e[0]
+
+
+# --------------------------------------------------------------------------------------
+# energy balance of the centred lossy boundary
+# --------------------------------------------------------------------------------------
+# At a centred boundary node the missing neighbour on each of its D = dir_x + dir_y + dir_z
+# axes takes the value of the present one (S_boundary, kernels3d.cu:645-650), i.e. the node
+# sits on a mirror plane. With weights w_i = 2^-D_i and edge weights W_ij = w_i c_ij (c_ij = 2
+# when the edge runs along one of i's mirrored axes, else 1) the scheme's Laplacian
+# (L p)_i = sum_j c_ij (p_j - p_i) is symmetric in a box room, and with lambda^2 = 1/3 the
+# update (1 + beta) p^{n+1} = lambda^2 S + (beta - 1) p^{n-1} (:652) reads
+#     p^{n+1} - 2 p^n + p^{n-1} = lambda^2 L p^n - beta (p^{n+1} - p^{n-1}).
+# Multiplying by w_i (p^{n+1}_i - p^{n-1}_i) and summing gives, exactly,
+#     E^{n+1/2} - E^{n-1/2} = -sum_i w_i beta_i (p^{n+1}_i - p^{n-1}_i)^2,
+#     E^{n+1/2} = sum_i w_i (p^{n+1}_i - p^n_i)^2 + lambda^2 sum_{edges} W_ij dp^{n+1} dp^n,
+# with beta_i = table[mat_i] * lambda * D_i (:639-640). A beta off by any factor, or taken per
+# node instead of per mirrored axis, breaks the identity.
+LAM = math.sqrt(1.0 / 3.0)
+
+
+def _centred_room():
+    """Box room whose six walls carry four materials (x walls 0 / 1, y walls 2, z walls 3)."""
+    X, Y, Z = 12, 10, 9
+    lab = rooms.box_room(X, Y, Z)
+    lab[:, :, 0] = 1
+    lab[:, :, X - 1] = 2
+    lab[:, 0, :] = lab[:, Y - 1, :] = 3
+    lab[0, :, :] = lab[Z - 1, :, :] = 4
+    pos, mat, counts = oracle.classify_kow(lab)
+    assert counts[2] == 0
+    return pos, mat
+
+
+def _weights(pos):
+    inside = (pos >> 7).astype(bool)
+    dirs = np.stack([(pos >> a) & 1 for a in range(3)]).astype(np.int64)  # [axis][z][y][x]
+    D = dirs.sum(axis=0)
+    w = np.where(inside, 2.0 ** (-D), 0.0)
+    edges = []  # (axis, W for the edge between node and its + neighbour along axis)
+    for ax in range(3):
+        npax = 2 - ax  # numpy axis of x (0) / y (1) / z (2)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[npax] = slice(None, -1)
+        hi[npax] = slice(1, None)
+        both = inside[tuple(lo)] & inside[tuple(hi)]
+        W_lo = w[tuple(lo)] * (1 + dirs[ax][tuple(lo)])
+        W_hi = w[tuple(hi)] * (1 + dirs[ax][tuple(hi)])
+        assert np.array_equal(W_lo[both], W_hi[both])  # symmetric edge weights
+        edges.append((tuple(lo), tuple(hi), np.where(both, W_lo, 0.0)))
+    return inside, D, w, edges
+
+
+def _energy(p1, p0, w, edges):
+    e = np.sum(w * (p1 - p0) ** 2)
+    for lo, hi, W in edges:
+        e += LAM * LAM * np.sum(W * (p1[lo] - p1[hi]) * (p0[lo] - p0[hi]))
+    return e
+
+
+def _fields(pos, mat, table, n, src, sig):
+    _, P, Pp = oracle.run(pos, mat, table, src, np.ones(1, np.int32), sig, src[:0], n,
+                          precision="f64", literal=oracle.SCHEME_KOWALCZYK, want_fields=True)
+    return P, Pp
+
+
+def test_centred_energy_conserved_rigid_walls():
+    pos, mat = _centred_room()
+    _, _, w, edges = _weights(pos)
+    src = np.array([[4, 5, 3]], np.int32)
+    sig = np.zeros((1, 60))
+    sig[0, 0] = 1.0
+    table = np.zeros((5, 20), np.float32)
+    Es = []
+    for n in range(2, 50, 8):
+        P, Pp = _fields(pos, mat, table, n, src, sig)
+        Es.append(_energy(P, Pp, w, edges))
+    Es = np.array(Es)
+    assert Es[0] > 0.1
+    assert np.max(np.abs(Es - Es[0])) < 1e-12 * Es[0]
+
+
+def test_centred_energy_balance_lossy_walls():
+    pos, mat = _centred_room()
+    inside, D, w, edges = _weights(pos)
+    betas = np.array([0.08, 0.35, 0.6, 0.2, 0.45], np.float32)
+    table = np.zeros((5, 20), np.float32)
+    table[:, 0] = betas
+    beta = betas.astype(np.float64)[mat] * LAM * D * inside  # beta = table * lambda * D
+    assert set(np.unique(D[inside])) == {0, 1, 2, 3}
+    src = np.array([[3, 4, 2]], np.int32)  # near a corner: faces, edges and corners absorb
+    sig = np.zeros((1, 60))
+    sig[0, 0] = 1.0
+    prev = None
+    for n in range(3, 40, 6):
+        P0, Pm = _fields(pos, mat, table, n, src, sig)  # p^n, p^{n-1}
+        P1, _ = _fields(pos, mat, table, n + 1, src, sig)
+        e_hi, e_lo = _energy(P1, P0, w, edges), _energy(P0, Pm, w, edges)
+        d2 = (P1 - Pm) ** 2
+        loss = -np.sum(w * beta * d2)
+        assert loss < 0
+        assert abs((e_hi - e_lo) - loss) < 1e-11 * e_lo
+        # the identity fails for a beta without the per-axis count D, or halved
+        for wrong in (betas.astype(np.float64)[mat] * LAM * inside, 0.5 * beta):
+            bad = -np.sum(w * wrong * d2)
+            assert abs((e_hi - e_lo) - bad) > 1e-6 * abs(loss)
+        if prev is not None:
+            assert e_hi < prev
+        prev = e_hi
