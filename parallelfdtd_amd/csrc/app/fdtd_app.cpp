@@ -36,6 +36,11 @@ void progress_tramp(int step, int n, float sec, void *user) {
 
 }  // namespace
 
+// loggerInit, src/logger.cpp: start an empty solver_log.txt in the working directory
+void loggerInit() {
+  std::wofstream logfile("solver_log.txt", std::fstream::out | std::fstream::trunc);
+}
+
 // ---------------------------------------------------------------------------------------
 // Position / SimulationParameters
 // ---------------------------------------------------------------------------------------
@@ -578,22 +583,32 @@ void App::runCapture() {
   check(pfdtd_begin_steps(ctx_, PFDTD_F32, (int)steps), "runCapture");
   current_step_ = 0;
   if (step_direction_ == 1) {
-    // the steps between two scheduled captures advance in one call (the two-level sweep where
-    // it applies), at most kCaptureSegment steps at a time so the interrupt is still polled;
-    // results equal the reference's step-by-step loop
-    constexpr unsigned int kCaptureSegment = 100;
+    // The steps between two scheduled captures advance in one call (the two-level sweep where
+    // it applies); results equal the reference's step-by-step loop. Segments also end at every
+    // multiple of PROGRESS_MOD (100) steps: the progress callback reports step 100 k there, as
+    // launchFDTD3dStep does for those steps (kernels3d.cu:476-479), and the interrupt is
+    // polled after every segment -- at most 100 steps apart, where the reference polls after
+    // every step (App.cpp:362-366), so an interrupt can let up to 99 more steps complete.
+    constexpr unsigned int kProgressMod = 100;
     const auto t0 = std::chrono::steady_clock::now();
+    auto seg_t0 = t0;
     unsigned int done = 0;
     while (done < steps) {
-      unsigned int stop = std::min(steps, done + kCaptureSegment);
+      unsigned int stop = std::min(steps, (done / kProgressMod + 1) * kProgressMod);
       for (unsigned int s : step_to_capture_)
         if (s > done && s < stop) stop = s;
       for (unsigned int s : mesh_to_capture_)
         if (s > done && s < stop) stop = s;
       check(pfdtd_steps(ctx_, (int)done, (int)(stop - done)), "runCapture");
+      const auto now = std::chrono::steady_clock::now();
+      const unsigned int first = done;
       done = stop;
       current_step_ = done;
       captureScheduled();
+      if (m_progress && first % kProgressMod == 0)
+        m_progress((int)first, (int)steps,
+                   (float)(std::chrono::duration<double>(now - seg_t0).count() / (stop - first)));
+      seg_t0 = now;
       if (m_interrupt && m_interrupt()) break;
     }
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -615,7 +630,13 @@ void App::runCapture() {
 void App::executeStep() {
   ensureContext();
   const auto t0 = std::chrono::steady_clock::now();
-  check(pfdtd_step(ctx_, (int)current_step_, step_direction_), "executeStep");
+  const unsigned int step = current_step_;
+  check(pfdtd_step(ctx_, (int)step, step_direction_), "executeStep");
+  // launchFDTD3dStep reports progress on steps that are multiples of PROGRESS_MOD
+  // (kernels3d.cu:476-479)
+  if (m_progress && step % 100 == 0)
+    m_progress((int)step, (int)m_parameters.getNumSteps(),
+               (float)std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   current_step_ += step_direction_;
   captureScheduled();
   time_per_step_ = (float)std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -20,6 +20,8 @@
 #include <string>
 #include <vector>
 
+#include "global_includes.h"  // logger, reflection2Admitance / admitance2Reflection
+
 struct pfdtd_ctx;
 
 // enum SrcType / InputType, src/base/SrcRec.h:33-34
@@ -44,9 +46,6 @@ struct Vec3i {
 };
 }  // namespace nv
 
-// reflection2Admitance / admitance2Reflection, src/global_includes.h:30-38
-inline float reflection2Admitance(float coef) { return (1.f - coef) / (1.f + coef); }
-inline float admitance2Reflection(float coef) { return reflection2Admitance(coef); }
 
 // Position / Source / Receiver, src/base/SrcRec.h
 class Position {
