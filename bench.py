@@ -13,9 +13,17 @@ Prints one JSON line:
   roofline  = algorithmic bytes per update-kernel launch (14 B/node FP32) / average update
               kernel duration measured with HIP events on the library's stream, vs 8 TB/s;
               `traffic` = measured HBM bytes per launch from the committed rocprofv3 counter
-              summary when one exists;
+              summary when one exists, and `achieved_measured` / `frac_measured` = those bytes
+              over the same launch time (the rate HBM actually moved);
+  serving   = p50/p99 latency of the continuous-batching pool on a Poisson stream of
+              impulse-response jobs of the C5 kind (rooms 64..512 voxels per side);
   cpu_baseline = the CPU oracle (a C restatement of the reference update loop) timed on this
-              host on a z-slab sample of the same room.
+              host on a z-slab sample of the same room, on every host thread this process may
+              use (OMP_NUM_THREADS caps it where set), with single-thread, literal-mode and
+              130x98x66 full-run figures beside it.
+
+`--bw-probe` prints the part's streaming ceilings instead (read, write, copy, triad, in-place
+update over three C3-sized arrays), measured by the library's own probe kernels.
 """
 from __future__ import annotations
 
@@ -74,40 +82,114 @@ def committed_traffic() -> float | None:
     return float(m.group(1)) if m else None
 
 
+def host_threads() -> dict:
+    """Host threads this process may use: `nproc` (the affinity mask), the machine's online
+    processors (what std::thread::hardware_concurrency reports, sysconf _SC_NPROCESSORS_ONLN)
+    and the OpenMP cap the environment sets, if any."""
+    nproc = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(omp) if omp.isdigit() and int(omp) > 0 else None
+    return {"nproc": nproc, "hardware_concurrency": os.cpu_count(), "omp_num_threads": cap,
+            "used": min(nproc, cap) if cap else nproc}
+
+
+def _oracle_rate(pos, mat, table, src, sig, rec, steps, threads, literal=0) -> tuple[float, float]:
+    import oracle
+
+    t0 = time.perf_counter()
+    oracle.run(pos, mat, table, src, np.array([1], np.int32), sig, rec, steps, threads=threads,
+               literal=literal)
+    dt = time.perf_counter() - t0
+    return pos.size * steps / dt / 1e9, dt
+
+
 def cpu_baseline(scene, steps: int = 300, nz: int = 66) -> dict:
-    """Oracle timed on a z-slab of the same room: the first nz-1 slices plus a closing shell."""
+    """Oracle timed on a z-slab of the same room (the first nz-1 slices plus a closing shell):
+    every usable host thread (the reported value), one thread (a 10-slice slab), and the
+    reference's literal material slot (kernels3d.cu:513) on every thread; plus the 130x98x66
+    box run in full (2000 steps) on the CPU and on the GPU."""
     import oracle
 
     lab = np.ascontiguousarray(scene.labels[:nz].copy())
     lab[nz - 1] = 2  # ceiling material closes the slab
     pos, mat, _ = oracle.classify(lab)
     pts = rooms.seeded_points(lab, rooms.SplitMix64(99), 2)
-    threads = min(16, len(os.sched_getaffinity(0)))
+    ht = host_threads()
+    threads = ht["used"]
+    table = scene.material_table()
     sig = scene.signals[:, :steps].astype(np.float32)
-    t0 = time.perf_counter()
-    oracle.run(pos, mat, scene.material_table(), pts[:1], np.array([1], np.int32), sig, pts[1:],
-               steps, threads=threads)
-    dt = time.perf_counter() - t0
+    rate, dt = _oracle_rate(pos, mat, table, pts[:1], sig, pts[1:], steps, threads)
+    # literal slot: every node reads table slot mat*20*octave = 0 at octave 0 (same cost)
+    lit_steps = max(1, steps // 3)
+    rate_lit, dt_lit = _oracle_rate(pos, mat, table, pts[:1], sig[:, :lit_steps], pts[1:],
+                                    lit_steps, threads, literal=1)
+    one = np.ascontiguousarray(lab[:11].copy())
+    one[10] = 2
+    pos1, mat1, _ = oracle.classify(one)
+    p1 = rooms.seeded_points(one, rooms.SplitMix64(98), 2)
+    one_steps = 40
+    rate1, dt1 = _oracle_rate(pos1, mat1, table, p1[:1], sig[:, :one_steps], p1[1:], one_steps, 1)
     Z, Y, X = lab.shape
-    return {"value": round(lab.size * steps / dt / 1e9, 4), "unit": "Gvoxel-updates/s",
+    c1 = rooms.scene_c1(n_steps=2000)
+    pc, mc, _ = oracle.classify(c1.labels)
+    t0 = time.perf_counter()
+    oracle.run(pc, mc, c1.material_table(), c1.src_xyz, c1.src_kind,
+               c1.signals.astype(np.float32), c1.rec_xyz, 2000, threads=threads)
+    c1_cpu = time.perf_counter() - t0
+    with Solver(0) as s:
+        s.load_scene(c1)
+        s.run(20, N.F32)
+        t0 = time.perf_counter()
+        s.run(2000, N.F32)
+        c1_gpu = time.perf_counter() - t0
+    return {"value": round(rate, 4), "unit": "Gvoxel-updates/s",
             "cores": threads, "kind": "port",
             "sample": f"{X}x{Y}x{Z} z-slab of the same room, {steps} FP32 steps, "
-                      f"OpenMP over z on {threads} threads, {dt:.1f} s"}
+                      f"OpenMP over z on {threads} threads, {dt:.1f} s",
+            "host": ht,
+            "single_thread": {"value": round(rate1, 4), "sample":
+                              f"{X}x{Y}x11 slab, {one_steps} steps, 1 thread, {dt1:.1f} s"},
+            "literal_mode": {"value": round(rate_lit, 4), "sample":
+                             f"the {X}x{Y}x{Z} slab, {lit_steps} steps, literal material slot "
+                             f"(kernels3d.cu:513), {threads} threads, {dt_lit:.1f} s"},
+            "c1_full_run": {"workload": "130x98x66 box, 2000 FP32 steps, 1 soft source, "
+                                        "1 receiver",
+                            "cpu_s": round(c1_cpu, 3), "gpu_s": round(c1_gpu, 4),
+                            "cpu_threads": threads,
+                            "cpu_gvox_per_s": round(c1.n_nodes * 2000 / c1_cpu / 1e9, 4),
+                            "gpu_gvox_per_s": round(c1.n_nodes * 2000 / c1_gpu / 1e9, 3)}}
 
 
-SERVE_SCALE = 2  # serving leg: rooms 32..256 voxels per side (the C5 stream at half size)
-SERVE_RATE = 20.0
+def bw_probe(bytes_per_array: int, iters: int) -> dict:
+    """Streaming ceilings of this part from the library's probe kernels."""
+    import ctypes as C
+
+    out = {}
+    for name, kind in (("read", N.PROBE_READ), ("write", N.PROBE_WRITE), ("copy", N.PROBE_COPY),
+                       ("triad", N.PROBE_TRIAD), ("update_in_place", N.PROBE_UPDATE)):
+        ms, moved = C.c_double(), C.c_double()
+        N.check(N.lib().pfdtd_hbm_probe(0, kind, bytes_per_array, iters, C.byref(ms),
+                                        C.byref(moved)), "pfdtd_hbm_probe")
+        gbs = moved.value / (ms.value * 1e-3) / 1e9
+        out[name] = {"ms_per_launch": round(ms.value, 4), "bytes_per_launch": moved.value,
+                     "GB_per_s": round(gbs, 1), "frac_of_8TBs": round(gbs / PEAK_GBS, 4)}
+        log(f"{name}: {gbs:.0f} GB/s ({ms.value:.3f} ms per launch)")
+    return out
 
 
-def serving_leg(jobs: int, rate: float) -> dict:
+SERVE_RATE = 2.5  # jobs/s: below the pool's capacity on this stream (about 3.9 jobs/s)
+
+
+def serving_leg(jobs: int, rate: float, scale: int = 1) -> dict:
     """p50/p99 impulse-response latency of the continuous-batching pool on a Poisson stream."""
     from parallelfdtd_amd import serve
 
-    rep = serve.serve(jobs, rate, scale=SERVE_SCALE, step_scale=1, log=log)
+    rep = serve.serve(jobs, rate, scale=scale, step_scale=1, log=log)
+    size = "" if scale == 1 else f" at 1/{scale} size"
     return {"jobs": jobs, "rate_jobs_per_s": rate,
-            "stream": f"C5 stream at 1/{SERVE_SCALE} size: rooms {64 // SERVE_SCALE}.."
-                      f"{512 // SERVE_SCALE} per side, 0-10 boxes, 1-6 materials, 1-16 sources, "
-                      f"2000-8000 steps, FP32/FP16 1:1",
+            "stream": f"C5 impulse-response stream{size}: the first {jobs} jobs, rooms "
+                      f"{64 // scale}..{512 // scale} voxels per side, 0-10 boxes, 1-6 materials, "
+                      f"1-16 sources, 2000-8000 steps, FP32/FP16 1:1, Poisson arrivals",
             "p50_latency_s": round(rep["latency_s"]["p50"], 4),
             "p99_latency_s": round(rep["latency_s"]["p99"], 4),
             "gvox_per_s": round(rep["gvox_per_s"], 2), "makespan_s": round(rep["makespan_s"], 2),
@@ -130,7 +212,18 @@ def main() -> int:
                     help="only the FP16-storage run (profiling); prints no bench line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 4 GiB device copy (counter calibration for rocprofv3)")
+    ap.add_argument("--bw-probe", action="store_true",
+                    help="only measure the HBM streaming ceilings (read/write/copy/triad/update)")
+    ap.add_argument("--probe-gib", type=float, default=9.0,
+                    help="bytes per probe array in GiB (the C3 FP32 field is 9.0 GiB)")
     args = ap.parse_args()
+
+    if args.bw_probe:
+        rep = {"probe": "pfdtd_hbm_probe (float4 per lane, 4 in flight)",
+               "bytes_per_array": int(args.probe_gib * 2**30), "iters": 10,
+               "results": bw_probe(int(args.probe_gib * 2**30), 10)}
+        print(json.dumps(rep), flush=True)
+        return 0
 
     if args.copy_calibration:
         import torch
@@ -187,6 +280,8 @@ def main() -> int:
             log(f"FP16 storage: {fp16}")
 
     traffic = committed_traffic() if args.scale == 1 else None
+    launch_s = 2 * kms * 1e-3
+    measured = traffic / launch_s / 1e9 if traffic else None
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -206,13 +301,17 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": traffic,
+                     "achieved_kind": "algorithmic bytes (14 B per node-update, 2 updates per "
+                                      "launch) / launch time",
+                     "achieved_measured": round(measured, 1) if measured else None,
+                     "frac_measured": round(measured / PEAK_GBS, 4) if measured else None,
                      "kernel": "srl_update2_kernel<float> (two time levels per launch)",
                      "launch_ms": round(2 * kms, 4), "ms_per_step": round(kms, 4),
                      "bytes_per_node_update": BYTES_PER_NODE[N.F32]},
         "fp16_storage": fp16,
     }
     if args.serve_jobs > 0:
-        out["serving"] = serving_leg(args.serve_jobs, args.serve_rate)
+        out["serving"] = serving_leg(args.serve_jobs, args.serve_rate, args.scale)
         log(f"serving: {out['serving']}")
     if not args.no_cpu:
         t0 = time.perf_counter()

@@ -226,6 +226,21 @@ int pfdtd_capture_field(pfdtd_ctx *ctx, float *out, int64_t capacity);
 /* Whole current and past fields in storage precision (raw bytes), for checks. */
 int pfdtd_get_fields_raw(pfdtd_ctx *ctx, void *p_out, void *p_past_out, int64_t capacity_bytes);
 
+/* HBM streaming probe: the read, write and read+write ceilings the update kernels are measured
+ * against. kind: PFDTD_PROBE_READ (sum of a), _WRITE (a = s), _COPY (b = a), _TRIAD
+ * (a = b + s*c), _UPDATE (a = s*a + b in place: the leapfrog update's access pattern). Each
+ * array has bytes_per_array bytes (rounded down to 16 KiB); one warm-up launch, then `iters`
+ * timed launches (HIP events). Outputs: average launch time and the bytes one launch reads and
+ * writes (arrays read + arrays written). The arrays are allocated and freed inside the call.
+ * No reference counterpart: a measurement aid of this build. */
+#define PFDTD_PROBE_READ 0
+#define PFDTD_PROBE_WRITE 1
+#define PFDTD_PROBE_COPY 2
+#define PFDTD_PROBE_TRIAD 3
+#define PFDTD_PROBE_UPDATE 4
+int pfdtd_hbm_probe(int device, int kind, int64_t bytes_per_array, int iters,
+                    double *ms_per_launch, double *bytes_moved_per_launch);
+
 /* Timing of the update kernels inside pfdtd_run: when enabled, every update launch is bracketed
  * by HIP events on the context's stream. update_ms_total is their summed device time and
  * update_launches the number of time steps they advanced (a two-level launch counts 2). */

@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["pfdtd_kernels.hip", "pfdtd_fused.hip", "pfdtd_capi.hip", "pfdtd_pool.hip",
-               "pfdtd_voxel.hip"]
+               "pfdtd_voxel.hip", "pfdtd_probe.hip"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

@@ -1418,4 +1418,50 @@ int pfdtd_set_poll_interval(pfdtd_ctx *c, int steps) {
   return PFDTD_OK;
 }
 
+int pfdtd_hbm_probe(int device, int kind, int64_t bytes_per_array, int iters,
+                    double *ms_per_launch, double *bytes_moved_per_launch) {
+  if (kind < PFDTD_PROBE_READ || kind > PFDTD_PROBE_UPDATE)
+    return fail(PFDTD_E_ARG, "probe kind %d not in 0..4", kind);
+  if (iters < 1 || !ms_per_launch || !bytes_moved_per_launch)
+    return fail(PFDTD_E_ARG, "iterations < 1 or null output");
+  const int64_t unit = (int64_t)sizeof(float4) * kProbeBlockElems;
+  const int64_t blocks = bytes_per_array / unit;
+  if (blocks < 1 || blocks > 0x7FFFFFFF)
+    return fail(PFDTD_E_ARG, "array of %lld bytes outside 16 KiB .. 32 TiB", (long long)bytes_per_array);
+  const size_t bytes = (size_t)(blocks * unit);
+  const int arrays = kind == PFDTD_PROBE_TRIAD ? 3 : (kind <= PFDTD_PROBE_WRITE ? 1 : 2);
+  const int moved = kind <= PFDTD_PROBE_WRITE ? 1 : (kind == PFDTD_PROBE_COPY ? 2 : 3);
+  HIP_TRY(hipSetDevice(device));
+  float4 *buf[3] = {nullptr, nullptr, nullptr};
+  float *sink = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  float ms = 0.f;
+  hipError_t e = hipStreamCreate(&st);
+  for (int i = 0; i < arrays && e == hipSuccess; ++i) {
+    e = hipMalloc((void **)&buf[i], bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(buf[i], 0, bytes, st);
+  }
+  if (e == hipSuccess) e = hipMalloc((void **)&sink, 256 * sizeof(float));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = launch_hbm_probe(kind, buf[0], buf[1], buf[2], sink, blocks, st);
+  if (e == hipSuccess) e = hipEventRecord(e0, st);
+  for (int it = 0; it < iters && e == hipSuccess; ++it)
+    e = launch_hbm_probe(kind, buf[0], buf[1], buf[2], sink, blocks, st);
+  if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (st) (void)hipStreamSynchronize(st);
+  for (float4 *q : buf) dfree(q);
+  dfree(sink);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "hbm probe: %s", hipGetErrorString(e));
+  *ms_per_launch = (double)ms / iters;
+  *bytes_moved_per_launch = (double)bytes * moved;
+  return PFDTD_OK;
+}
+
 }  // extern "C"

@@ -260,6 +260,11 @@ hipError_t launch_transparent(bool f64, const void *in, const float *h, int n_ir
 // Centred-scheme coefficient tables: entry (D, m), D = number of DIR bits (0..3).
 void build_centred_tables(const float *table20, int n_mat, unsigned octave, double lambda,
                           CoefF *cf, CoefD *cd);
+// HBM streaming probe (pfdtd_probe.hip): one launch streams `blocks` * kProbeBlockElems float4
+// elements of each array the kind uses (a, then b, c; sink takes nothing in practice).
+constexpr int64_t kProbeBlockElems = 1024;
+hipError_t launch_hbm_probe(int kind, float4 *a, float4 *b, float4 *c, float *sink,
+                            int64_t blocks, hipStream_t st);
 // Gathers a plane of the field (unscaled FP32) and of the position bytes.
 hipError_t launch_capture_slice(int precision, const void *field, const uint8_t *pos, int X,
                                 int Y, int Z, int64_t slice, int orientation, int scale_exp,

@@ -22,6 +22,7 @@ SRC_HARD, SRC_SOFT, SRC_TRANSPARENT = 0, 1, 2
 SRL_FORWARD, SHARED, SRL = 0, 1, 2  # enum UpdateType
 MATSLOT_INTENDED, MATSLOT_LITERAL = 0, 1
 SOFT_ADD, SOFT_OVERWRITE = 0, 2
+PROBE_READ, PROBE_WRITE, PROBE_COPY, PROBE_TRIAD, PROBE_UPDATE = 0, 1, 2, 3, 4
 
 INTERRUPT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
 PROGRESS_CB = C.CFUNCTYPE(None, C.c_int, C.c_int, C.c_float, C.c_void_p)
@@ -84,6 +85,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_update_tiling": [_p, C.c_int, C.c_int],
     "pfdtd_set_time_levels": [_p, C.c_int, C.c_int],
     "pfdtd_set_sweep_shape": [_p, C.c_int],
+    "pfdtd_hbm_probe": [C.c_int, C.c_int, _i64, C.c_int, _pdbl, _pdbl],
     "pfdtd_pool_create": [C.c_int, _i64, C.POINTER(_p)],
     "pfdtd_pool_destroy": [_p],
     "pfdtd_pool_set_options": [_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],

@@ -1,8 +1,9 @@
 """Python host object over one C-ABI context (``pfdtd_ctx``).
 
 This is the thin host layer the tests, the benchmark and the serving loop use; the
-reference-shaped ``App`` facade (``parallelfdtd_amd.app``) sits on top of it. Arrays cross
-the boundary as C-contiguous numpy buffers; all compute happens in libpfdtd.so.
+reference-shaped ``FDTD::App`` facade (``csrc/app``, ``libpfdtd_app.so`` / ``libPyFDTD``)
+calls the same C ABI from C++. Arrays cross the boundary as C-contiguous numpy buffers; all
+compute happens in libpfdtd.so.
 """
 from __future__ import annotations
 

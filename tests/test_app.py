@@ -140,32 +140,76 @@ def test_run_simulation_matches_oracle(gpu, double):
 
 @pytest.mark.gpu
 def test_run_capture_writes_slices_and_meshes(gpu, tmp_path):
-    n_steps = 40
+    n_steps = 60
     app, lab, betas = _room_app(fs=40000, n_steps=n_steps)
     app.addSource(0.15, 0.12, 0.10, 1, 1, 0)
     app.addReceiver(0.2, 0.15, 0.12)
     app.setCaptureDirectory(str(tmp_path))
-    app.addSliceToCapture(10, 20, 0)
-    app.addSliceToCapture(12, 30, 1)
-    app.addSliceToCapture(7, 30, 2)
-    app.addSliceToCapture(999, 30, 0)  # out of range: logged and skipped, as the reference does
+    app.addSliceToCapture(10, 40, 0)
+    app.addSliceToCapture(12, 50, 1)
+    app.addSliceToCapture(7, 50, 2)
+    app.addSliceToCapture(999, 50, 0)  # out of range: logged and skipped, as the reference does
     app.addMeshToCapture(25)
     app.runCapture()
     ref = _oracle_responses(app, lab, betas, n_steps, "f32")
     assert np.array_equal(app.getResponse(0), ref[0])
     files = sorted(os.listdir(tmp_path))
-    assert files == ["capture_0_20_10.tga", "capture_1_30_12.tga", "capture_2_30_7.tga"]
+    assert files == ["capture_0_40_10.tga", "capture_1_50_12.tga", "capture_2_50_7.tga"]
     X, Y, Z = lab.shape[2], lab.shape[1], lab.shape[0]
-    raw = (tmp_path / "capture_0_20_10.tga").read_bytes()
+    raw = (tmp_path / "capture_0_40_10.tga").read_bytes()
     assert len(raw) == 18 + 4 * X * Y and raw[2] == 2 and raw[16] == 32
     assert app.getNumberOfMeshCaptures() == 1
     pos, mat, _ = oracle.classify(lab)
     src = np.array([app.getSourceElementCoordinates(0)], np.int32)
-    sig = np.array([[app.getSourceSample(0, n) for n in range(25)]])
-    _, P, _ = oracle.run(pos, mat, np.repeat(betas[:, None], 20, axis=1), src, np.array([1]), sig,
-                         src[:0], 25, want_fields=True)
-    assert np.array_equal(app.getMeshCaptureAt(0), P)
+    sig = np.array([[app.getSourceSample(0, n) for n in range(50)]])
+
+    def field_after(n):
+        return oracle.run(pos, mat, np.repeat(betas[:, None], 20, axis=1), src, np.array([1]),
+                          sig[:, :n], src[:0], n, want_fields=True)[1]
+
+    assert np.array_equal(app.getMeshCaptureAt(0), field_after(25))
+    # pixel values: the oracle's slices through the reference's dB colour map
+    lit = 0
+    for name, step, sl, orient in (("capture_0_40_10.tga", 40, 10, 0),
+                                   ("capture_1_50_12.tga", 50, 12, 1),
+                                   ("capture_2_50_7.tga", 50, 7, 2)):
+        p, q = oracle.capture_slice(field_after(step), pos, sl, orient)
+        raw = (tmp_path / name).read_bytes()
+        h, w = p.shape
+        assert raw[12] | raw[13] << 8 == w and raw[14] | raw[15] << 8 == h
+        got = np.frombuffer(raw[18:], np.uint8).reshape(h, w, 4)
+        exp = _tga_pixels(p, q, 60.0)
+        assert np.array_equal(got[..., [2, 3]], exp[..., [2, 3]])  # red and alpha
+        d = np.abs(got.astype(int) - exp.astype(int))
+        # green/blue: equal, up to one level where log10 of a float rounds differently
+        assert d.max() <= 1 and np.count_nonzero(d) <= max(2, d.size // 1000), name
+        assert np.count_nonzero((exp[..., 0] == 255) & (exp[..., 1] == 255)) > 0  # walls
+        lit += np.count_nonzero((exp[..., 2] == 0) & ((exp[..., 0] > 0) | (exp[..., 1] > 0)))
+    assert lit > 100  # the wave is on the captured planes
     app.close()
+
+
+def _tga_pixels(p, q, capture_db):
+    """(h, w, 4) BGRA bytes of App::saveBitmap (App.cpp:492-567) for a captured plane p with
+    position bytes q: boundary voxels (inside, K != 6) white; elsewhere green for positive and
+    blue for negative pressure, c = (log10(p^2) + dB)/dB with dB = capture_db/10, c < 0 -> 0.
+    Above full scale (c > 1) the reference's float-to-byte conversion is undefined; the
+    build saturates at 255."""
+    p = np.asarray(p, np.float32)
+    dB = np.float32(capture_db / 10.0)
+    with np.errstate(divide="ignore"):
+        c = (np.log10(p * p) + dB) / dB
+    c = np.clip(np.where(c > 0, c, np.float32(0)), 0, 1).astype(np.float32)
+    pos_sign = p > 0
+    g = np.where(pos_sign, c, 0).astype(np.float32) * np.float32(255)
+    b = np.where(~pos_sign, c, 0).astype(np.float32) * np.float32(255)
+    out = np.zeros(p.shape + (4,), np.uint8)
+    out[..., 0] = b.astype(np.uint8)
+    out[..., 1] = g.astype(np.uint8)
+    out[..., 3] = 255
+    wall = ((q >> 7) == 1) & ((q & 0x7F) != 6)
+    out[wall, 0:3] = 255
+    return out
 
 
 @pytest.mark.gpu

@@ -277,3 +277,78 @@ def test_steps_match_run_and_reject_overrun(gpu):
             s.steps(30, 12)
         with pytest.raises(N.PfdtdError):
             s.steps(-1, 2)
+
+
+def _app_step_calls(n_forward, back, n_steps):
+    """(step, direction) per pfdtd_step call as App::executeStep issues them
+    (App.cpp:407-436): current_step_ += direction after each call; invertTime flips the
+    direction (App.cpp:403-405)."""
+    steps, dirs = [], []
+    cur, d = 0, 1
+    for count, direction in ((n_forward, 1), (back, -1), (n_steps, 1)):
+        d = direction
+        for _ in range(count):
+            if not 0 <= cur < n_steps:
+                break
+            steps.append(cur)
+            dirs.append(d)
+            cur += d
+    return steps, dirs
+
+
+@pytest.mark.parametrize("prec,storage,scale", [(N.F32, oracle.STORE_F32, 0),
+                                                (N.F16, oracle.STORE_F16, 9)])
+def test_step_directions_match_oracle(gpu, prec, storage, scale):
+    """Forward, backward, forward pfdtd_step sequences against the oracle's single-step mode
+    (orc_step_calls_f): the fields swap only while the direction is unchanged
+    (kernels3d.cu:462-465), and each call writes response sample `step`."""
+    sc = rooms.scene_random_small(54, dims=(70, 26, 17), n_mat=3, n_boxes=4, n_src=3, n_rec=5,
+                                  n_steps=60)
+    pos, mat, _ = oracle.classify(sc.labels)
+    steps, dirs = _app_step_calls(25, 6, 60)
+    steps2, dirs2 = _app_step_calls(10, 1, 60)  # a single backward step
+    for st, dr in ((steps, dirs), (steps2, dirs2)):
+        ref, Q, Qp = oracle.step_calls(pos, mat, sc.material_table(), sc.src_xyz, sc.src_kind,
+                                       sc.signals, sc.rec_xyz, sc.n_steps, st, dr,
+                                       storage=storage, scale_exp=scale)
+        raw = np.float32 if prec == N.F32 else np.float16
+        with Solver() as s:
+            s.load_scene(sc)
+            if scale:
+                s.prepare_fields(prec, scale)
+            s.begin_steps(sc.n_steps, prec)
+            for n, d in zip(st, dr):
+                s.step(n, d)
+            got = s.responses(np.float32)
+            P, Pp = s.fields_raw(raw)
+        assert np.abs(ref).max() > 0
+        assert np.array_equal(got, ref)
+        assert np.array_equal(P.astype(np.float32), Q) and np.array_equal(Pp.astype(np.float32), Qp)
+
+
+@pytest.mark.parametrize("prec,storage,scale", [(N.F32, oracle.STORE_F32, 0),
+                                                (N.F16, oracle.STORE_F16, 7)])
+def test_capture_slice_matches_oracle_capture(gpu, prec, storage, scale):
+    """pfdtd_capture_slice against the oracle's captureSliceFast restatement
+    (orc_capture_slice_f, visualizationUtils.cu:130-254, :500-527) on the oracle's field:
+    every slice of every orientation, and the out-of-range slices it refuses."""
+    sc = rooms.scene_random_small(32, dims=(23, 17, 13), n_steps=30)
+    pos, mat, _ = oracle.classify(sc.labels)
+    _, P_ref, _ = oracle_for(sc, pos, mat, storage=storage, scale_exp=scale, want_fields=True)
+    P_ref = P_ref * np.float32(2.0 ** -scale)
+    Z, Y, X = sc.labels.shape
+    with Solver() as s:
+        s.load_scene(sc)
+        if scale:
+            s.prepare_fields(prec, scale)
+        s.run(sc.n_steps, prec)
+        for orient, lim in ((0, Z), (1, Y), (2, X)):
+            for sl in range(lim):
+                p, q = s.capture_slice(sl, orient)
+                ep, eq = oracle.capture_slice(P_ref, pos, sl, orient)
+                assert np.array_equal(p, ep) and np.array_equal(q, eq), (orient, sl)
+            assert oracle.capture_slice(P_ref, pos, lim, orient) is None
+            with pytest.raises(N.PfdtdError):
+                s.capture_slice(lim, orient)
+            with pytest.raises(N.PfdtdError):
+                s.capture_slice(-1, orient)
