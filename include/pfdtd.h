@@ -221,6 +221,20 @@ int pfdtd_get_response_shape(pfdtd_ctx *ctx, int *n_rec, int *n_steps);
  * skips the capture). */
 int pfdtd_capture_slice(pfdtd_ctx *ctx, int64_t slice, int orientation, float *p_out,
                         uint8_t *pos_out);
+/* Largest plane, in elements, that pfdtd_capture_slice_noalloc accepts (every plane of a
+ * 2048 x 1536 x 768 room fits). */
+#define PFDTD_STATIC_CAPTURE_MAX 4194304
+/* pfdtd_capture_slice without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (tight capture or serving loops, harnesses
+ * that forbid hipMalloc). The plane is staged in a static device buffer of
+ * PFDTD_STATIC_CAPTURE_MAX * 5 bytes (20 MiB) that every GPU the library runs on holds until
+ * the process ends; a plane larger than PFDTD_STATIC_CAPTURE_MAX elements is refused with
+ * PFDTD_E_ARG. Every call on a GPU shares that GPU's buffer, so calls that overlap on one GPU
+ * (from several host threads or contexts) race. The buffer keeps the last call's contents;
+ * nothing in it needs clearing, since each call writes the whole plane before copying it out.
+ * Same arguments, results and errors as pfdtd_capture_slice. */
+int pfdtd_capture_slice_noalloc(pfdtd_ctx *ctx, int64_t slice, int orientation, float *p_out,
+                                uint8_t *pos_out);
 /* Whole current field as unscaled FP32: captureMesh, visualizationUtils.cu:111-128. */
 int pfdtd_capture_field(pfdtd_ctx *ctx, float *out, int64_t capacity);
 /* Whole current and past fields in storage precision (raw bytes), for checks. */

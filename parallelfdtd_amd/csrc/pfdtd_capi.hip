@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -224,6 +225,11 @@ void build_centred_tables(const float *table20, int n_mat, unsigned octave, doub
 
 }  // namespace pfdtd
 
+// Staging planes of pfdtd_capture_slice_noalloc: FP32 values (PFDTD_STATIC_CAPTURE_MAX * 4
+// bytes) followed by the position bytes. Each GPU holds its own copy at its own address; host
+// code reaches it only through the address hipGetSymbolAddress returns on that device.
+static __device__ __align__(256) unsigned char scratch[(size_t)PFDTD_STATIC_CAPTURE_MAX * 5];
+
 namespace {
 
 int check_ctx(pfdtd_ctx *ctx) {
@@ -232,6 +238,28 @@ int check_ctx(pfdtd_ctx *ctx) {
   if (e != hipSuccess)
     return fail(PFDTD_E_HIP, "hipSetDevice(%d): %s", ctx->device, hipGetErrorString(e));
   return PFDTD_OK;
+}
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // device address of `scratch`, per device (null: unresolved)
+
+// Resolves `scratch` on `device`, which must be current; once per device (pfdtd_ctx_create).
+int resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return PFDTD_OK;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(PFDTD_E_HIP, "static capture buffer on device %d: %s", device,
+                hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return PFDTD_OK;
+}
+
+void *scratch_on(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
 }
 
 int64_t lin_idx(const pfdtd_ctx *c, const int32_t *q) {
@@ -612,6 +640,8 @@ int pfdtd_ctx_create(int device, pfdtd_ctx **out) {
   if (device < 0 || device >= n)
     return fail(PFDTD_E_ARG, "device %d not present (%d devices)", device, n);
   HIP_TRY(hipSetDevice(device));
+  int rs = resolve_scratch(device);
+  if (rs) return rs;
   pfdtd_ctx *c = new pfdtd_ctx();
   c->device = device;
   hipError_t e = hipStreamCreate(&c->stream);
@@ -1247,8 +1277,11 @@ int pfdtd_get_responses_f32(pfdtd_ctx *c, float *out, int64_t capacity) {
   return PFDTD_OK;
 }
 
-int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_out,
-                        uint8_t *pos_out) {
+namespace {
+
+// Checks of pfdtd_capture_slice(_noalloc); *n = elements of the plane.
+int capture_plane(pfdtd_ctx *c, int64_t slice, int orientation, const float *p_out,
+                  const uint8_t *pos_out, int64_t *n) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (c->lgB) return fail(PFDTD_E_STATE, "capture of batched fields is not supported (use pfdtd_get_fields_raw)");
@@ -1263,7 +1296,17 @@ int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_o
   }
   if (slice < 0 || slice >= lim)
     return fail(PFDTD_E_ARG, "slice %lld out of bounds %lld, no capture made", (long long)slice, (long long)lim);
-  const int64_t n = U * V;
+  *n = U * V;
+  return PFDTD_OK;
+}
+
+}  // namespace
+
+int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_out,
+                        uint8_t *pos_out) {
+  int64_t n = 0;
+  int rc = capture_plane(c, slice, orientation, p_out, pos_out, &n);
+  if (rc) return rc;
   float *d_p = nullptr;
   uint8_t *d_q = nullptr;
   HIP_TRY(hipMalloc((void **)&d_p, sizeof(float) * n));
@@ -1277,6 +1320,28 @@ int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_o
   (void)hipFree(d_p);
   if (d_q) (void)hipFree(d_q);
   if (e != hipSuccess) return fail(PFDTD_E_HIP, "capture_slice: %s", hipGetErrorString(e));
+  return PFDTD_OK;
+}
+
+int pfdtd_capture_slice_noalloc(pfdtd_ctx *c, int64_t slice, int orientation, float *p_out,
+                                uint8_t *pos_out) {
+  int64_t n = 0;
+  int rc = capture_plane(c, slice, orientation, p_out, pos_out, &n);
+  if (rc) return rc;
+  if (n > PFDTD_STATIC_CAPTURE_MAX)
+    return fail(PFDTD_E_ARG, "plane of %lld elements exceeds the static capture buffer (%lld)",
+                (long long)n, (long long)PFDTD_STATIC_CAPTURE_MAX);
+  unsigned char *base = (unsigned char *)scratch_on(c->device);
+  if (!base) return fail(PFDTD_E_STATE, "static capture buffer not resolved on device %d", c->device);
+  float *d_p = (float *)base;
+  uint8_t *d_q = base + (size_t)PFDTD_STATIC_CAPTURE_MAX * sizeof(float);
+  hipError_t e = launch_capture_slice(c->precision, c->d_P, c->d_pos, (int)c->X, (int)c->Y,
+                                      (int)c->Z, slice, orientation, c->scale_exp, d_p, d_q,
+                                      c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(p_out, d_p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pos_out, d_q, (size_t)n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(PFDTD_E_HIP, "capture_slice_noalloc: %s", hipGetErrorString(e));
   return PFDTD_OK;
 }
 

@@ -22,6 +22,7 @@ SRC_HARD, SRC_SOFT, SRC_TRANSPARENT = 0, 1, 2
 SRL_FORWARD, SHARED, SRL = 0, 1, 2  # enum UpdateType
 MATSLOT_INTENDED, MATSLOT_LITERAL = 0, 1
 SOFT_ADD, SOFT_OVERWRITE = 0, 2
+STATIC_CAPTURE_MAX = 4194304  # PFDTD_STATIC_CAPTURE_MAX
 PROBE_READ, PROBE_WRITE, PROBE_COPY, PROBE_TRIAD, PROBE_UPDATE = 0, 1, 2, 3, 4
 
 INTERRUPT_CB = C.CFUNCTYPE(C.c_int, C.c_void_p)
@@ -72,6 +73,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_get_responses_f64": [_p, _p, _i64],
     "pfdtd_get_response_shape": [_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "pfdtd_capture_slice": [_p, _i64, C.c_int, _p, _p],
+    "pfdtd_capture_slice_noalloc": [_p, _i64, C.c_int, _p, _p],
     "pfdtd_capture_field": [_p, _p, _i64],
     "pfdtd_get_fields_raw": [_p, _p, _p, _i64],
     "pfdtd_set_kernel_timing": [_p, C.c_int],

@@ -237,6 +237,15 @@ class Solver:
         self._c("pfdtd_capture_slice", slice_idx, orientation, N.ptr(p), N.ptr(q))
         return p, q
 
+    def capture_slice_noalloc(self, slice_idx: int, orientation: int) -> tuple[np.ndarray, np.ndarray]:
+        """capture_slice staged in the library's static device buffer (no device allocation)."""
+        X, Y, Z = self.dims
+        shape = {0: (Y, X), 1: (Z, X), 2: (Z, Y)}[orientation]
+        p = np.zeros(shape, np.float32)
+        q = np.zeros(shape, np.uint8)
+        self._c("pfdtd_capture_slice_noalloc", slice_idx, orientation, N.ptr(p), N.ptr(q))
+        return p, q
+
     def capture_field(self) -> np.ndarray:
         X, Y, Z = self.dims
         out = np.zeros((Z, Y, X), np.float32)

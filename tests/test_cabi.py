@@ -110,6 +110,7 @@ def test_binding_constants_match_header():
              "PFDTD_OK": native.OK, "PFDTD_E_ARG": native.E_ARG, "PFDTD_E_HIP": native.E_HIP,
              "PFDTD_E_NOMEM": native.E_NOMEM, "PFDTD_E_STATE": native.E_STATE,
              "PFDTD_E_DOMAIN": native.E_DOMAIN, "PFDTD_SRC_HARD": native.SRC_HARD,
-             "PFDTD_SRC_SOFT": native.SRC_SOFT, "PFDTD_SRC_TRANSPARENT": native.SRC_TRANSPARENT}
+             "PFDTD_SRC_SOFT": native.SRC_SOFT, "PFDTD_SRC_TRANSPARENT": native.SRC_TRANSPARENT,
+             "PFDTD_STATIC_CAPTURE_MAX": native.STATIC_CAPTURE_MAX}
     for name, value in pairs.items():
         assert d[name] == value, name
