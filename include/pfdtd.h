@@ -50,9 +50,12 @@ extern "C" {
 #define PFDTD_SOFT_OVERWRITE 2   /* soft source overwrites, as addSample does (cudaMesh.h:362-366) */
 
 /* ---- block shape of the two-level sweep (pfdtd_set_sweep_shape) -------------------- */
-#define PFDTD_SWEEP_AUTO 0   /* wide for FP32 rooms of at least 960 columns, else column */
+#define PFDTD_SWEEP_AUTO 0   /* FP32: wide on rooms of at least 960 columns, else column;
+                                16-bit storage: pair when X is even, else column */
 #define PFDTD_SWEEP_COLUMN 1 /* blocks of 1 x 4 wave tiles (60 columns x 4 tiles of rows) */
 #define PFDTD_SWEEP_WIDE 2   /* FP32: blocks of 4 x 4 wave tiles (240 columns, 1024 threads) */
+#define PFDTD_SWEEP_PAIR 3   /* 16-bit storage, X even: two columns per lane (124-column wave
+                                tiles, one 32-bit access per pair); otherwise as AUTO */
 
 typedef struct pfdtd_ctx pfdtd_ctx;
 
@@ -275,8 +278,8 @@ int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
  * Results are bit-identical to single steps. No reference counterpart: the reference's loop
  * (kernels3d.cu:83-191) launches one update per step. */
 int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
-/* Block shape of the two-level sweep: PFDTD_SWEEP_AUTO (default), _COLUMN or _WIDE (FP32 only;
- * other precisions always use columns). The wave tiles of a block step z together so their
+/* Block shape of the two-level sweep: PFDTD_SWEEP_AUTO (default), _COLUMN, _WIDE (FP32 only)
+ * or _PAIR (16-bit storage on rooms of even X; elsewhere it acts as AUTO). The wave tiles of a block step z together so their
  * shared halo columns and rows meet in cache: wide blocks share more of them but leave up to
  * three tiles past the grid's last column, so they pay off on wide rooms. Results do not
  * depend on it. No reference counterpart (a tuning knob of the two-level sweep). */

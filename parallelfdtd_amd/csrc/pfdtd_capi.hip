@@ -71,6 +71,7 @@ struct pfdtd_ctx {
   uint8_t *d_pos = nullptr, *d_mat = nullptr;
   uint16_t *d_pm = nullptr;  // pos | mat << 8, built for the two-level update (pm_ok: current)
   uint32_t *d_air_rows = nullptr;  // its air-row bits (built with it; null: not available)
+  uint32_t *d_air_rows_p = nullptr;  // the same for the two-column sweep's tiles (X even)
   bool pm_ok = false;
   int64_t dom_cap = 0;  // voxels the position/material buffers hold
   bool has_domain = false;
@@ -351,6 +352,7 @@ int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
   }
   c->pm_ok = false;
   dfree(c->d_air_rows);  // sized by the grid's shape, not only its voxel count
+  dfree(c->d_air_rows_p);
   // fields sized for the previous grid are dropped
   free_fields(c);
   if (!c->d_pos) {
@@ -549,10 +551,11 @@ bool fused_eligible(const pfdtd_ctx *c) {
 }
 
 template <typename S, typename CT, typename C>
-Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step) {
+Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step,
+                                bool pair) {
   Fused2Args<S, CT, C> a;
   a.pm = c->d_pm;
-  a.air_rows = c->d_air_rows;
+  a.air_rows = pair ? c->d_air_rows_p : c->d_air_rows;
   a.A = (const S *)c->d_P;
   a.B = (const S *)c->d_Pp;
   a.Cn = (S *)c->d_Pp2;
@@ -583,17 +586,17 @@ Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C la
 hipError_t launch_update2_any(pfdtd_ctx *c, const FusedGeom &g, int step) {
   switch (c->precision) {
   case PFDTD_F32:
-    return launch_update2_f32(g, fused_args<float>(c, c->d_coef_f, c->air_f, c->lam2_f, step),
-                              c->stream);
+    return launch_update2_f32(
+        g, fused_args<float>(c, c->d_coef_f, c->air_f, c->lam2_f, step, g.pair), c->stream);
   case PFDTD_F64:
-    return launch_update2_f64(g, fused_args<double>(c, c->d_coef_d, c->air_d, c->lam2_d, step),
-                              c->stream);
+    return launch_update2_f64(
+        g, fused_args<double>(c, c->d_coef_d, c->air_d, c->lam2_d, step, g.pair), c->stream);
   case PFDTD_F16:
-    return launch_update2_f16(g, fused_args<h16>(c, c->d_coef_f, c->air_f, c->lam2_f, step),
-                              c->stream);
+    return launch_update2_f16(
+        g, fused_args<h16>(c, c->d_coef_f, c->air_f, c->lam2_f, step, g.pair), c->stream);
   case PFDTD_BF16:
-    return launch_update2_bf16(g, fused_args<b16>(c, c->d_coef_f, c->air_f, c->lam2_f, step),
-                               c->stream);
+    return launch_update2_bf16(
+        g, fused_args<b16>(c, c->d_coef_f, c->air_f, c->lam2_f, step, g.pair), c->stream);
   default:
     return hipErrorInvalidValue;
   }
@@ -666,6 +669,7 @@ int pfdtd_ctx_destroy(pfdtd_ctx *c) {
   dfree(c->d_mat);
   dfree(c->d_pm);
   dfree(c->d_air_rows);
+  dfree(c->d_air_rows_p);
   dfree(c->d_scan);
   dfree(c->d_coef_f);
   dfree(c->d_coef_d);
@@ -1053,16 +1057,19 @@ static int ensure_pm(pfdtd_ctx *c, bool *fused) {
   }
   HIP_TRY(launch_pack_pm(c->d_pos, c->d_mat, c->d_pm, c->N, c->stream));
   // the air-row bits only save loads: without room for them the sweep reads every row
-  const size_t words = air_rows_words((int)c->X, (int)c->Y, (int)c->Z);
-  if (!c->d_air_rows && hipMalloc((void **)&c->d_air_rows, sizeof(uint32_t) * words) != hipSuccess) {
-    (void)hipGetLastError();
-    c->d_air_rows = nullptr;
-  }
-  if (c->d_air_rows &&
-      launch_air_rows(c->d_pm, c->d_air_rows, (int)c->X, (int)c->Y, (int)c->Z, c->scheme,
-                      c->stream) != hipSuccess) {
-    (void)hipGetLastError();
-    dfree(c->d_air_rows);
+  for (int pair = 0; pair < 2; ++pair) {
+    uint32_t *&rows = pair ? c->d_air_rows_p : c->d_air_rows;
+    if (pair && c->X % 2 != 0) continue;  // the two-column sweep needs X even
+    const size_t words = air_rows_words((int)c->X, (int)c->Y, (int)c->Z, pair != 0);
+    if (!rows && hipMalloc((void **)&rows, sizeof(uint32_t) * words) != hipSuccess) {
+      (void)hipGetLastError();
+      rows = nullptr;
+    }
+    if (rows && launch_air_rows(c->d_pm, rows, (int)c->X, (int)c->Y, (int)c->Z, c->scheme,
+                                pair != 0, c->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      dfree(rows);
+    }
   }
   c->pm_ok = true;
   return PFDTD_OK;
@@ -1416,8 +1423,9 @@ int pfdtd_set_time_levels(pfdtd_ctx *c, int levels, int zchunk) {
 
 int pfdtd_set_sweep_shape(pfdtd_ctx *c, int shape) {
   if (!c) return fail(PFDTD_E_ARG, "null context");
-  if (shape != PFDTD_SWEEP_AUTO && shape != PFDTD_SWEEP_COLUMN && shape != PFDTD_SWEEP_WIDE)
-    return fail(PFDTD_E_ARG, "sweep shape %d not AUTO (0), COLUMN (1) or WIDE (2)", shape);
+  if (shape != PFDTD_SWEEP_AUTO && shape != PFDTD_SWEEP_COLUMN && shape != PFDTD_SWEEP_WIDE &&
+      shape != PFDTD_SWEEP_PAIR)
+    return fail(PFDTD_E_ARG, "sweep shape %d not AUTO (0), COLUMN (1), WIDE (2) or PAIR (3)", shape);
   c->fshape = shape;
   return PFDTD_OK;
 }

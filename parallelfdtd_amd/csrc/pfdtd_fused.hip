@@ -40,6 +40,7 @@
 namespace pfdtd {
 
 constexpr int kFO = 60;       // owned columns per wave tile
+constexpr int kPO = 124;      // owned columns per wave tile of the two-column sweep
 constexpr int kFListCap = 32; // per-wave list capacity for sources / receivers (LDS)
 
 __device__ __forceinline__ bool in_box(const int32_t *q, int x0, int x1, int y0, int y1, int z0,
@@ -573,25 +574,386 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
   }
 }
 
+// ---- Two x-columns per lane (16-bit storage) ---------------------------------------------------
+// With one column per lane the 16-bit sweep issues one 2-byte load per voxel row: it is bound by
+// load issue and latency, not by bytes. Here lane l holds the adjacent columns xb-2+2l and
+// xb-1+2l and moves them with one 32-bit access, half the vector-memory instructions per byte:
+// a wave covers 128 columns, lanes 1..62 own 124 of them and lanes 0 and 63 hold the two halo
+// columns on either side. X must be even, so that every pair starts on a 4-byte boundary (rows
+// and slices of even length). The x-neighbours of a column are the other half of its own pair
+// and one half of the neighbouring lane's pair (a DPP shift); everything else -- the order of
+// the sums, the coefficient gathers, the rounding, injection and gather -- is the one-column
+// sweep's, so the results are the same bits.
+// Two adjacent columns of one row, as FP32 values (a plain aggregate: HIP's vector types are
+// unions, which keep the row queues from being promoted to registers).
+struct Pair2 {
+  float x, y;
+};
+
+// One column of a pair chosen per lane. The register barrier keeps the compiler from turning the
+// select of two loads into one load through a selected address, which would put the row queue
+// in scratch memory.
+__device__ __forceinline__ float pick(const Pair2 &p, bool hi) {
+  float lo = p.x, h = p.y;
+  asm("" : "+v"(lo), "+v"(h));
+  return hi ? h : lo;
+}
+
+template <typename S>
+__device__ __forceinline__ float lo_half(uint32_t u) {
+  return Fmt<S>::val(S{(uint16_t)(u & 0xFFFFu)});
+}
+template <typename S>
+__device__ __forceinline__ float hi_half(uint32_t u) {
+  return Fmt<S>::val(S{(uint16_t)(u >> 16)});
+}
+template <typename S>
+__device__ __forceinline__ uint32_t pack_halves(float a, float b) {
+  return (uint32_t)Fmt<S>::bits(a) | ((uint32_t)Fmt<S>::bits(b) << 16);
+}
+__device__ __forceinline__ uint32_t bld_pair(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)(e * 2u), 0, 0);
+}
+__device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(e * 2u), 0, 0);
+}
+
+// Rows per wave tile and waves per block (a column of wave tiles) of the two-column sweep.
+// Measured on the 2048x1536x768 room: 4-row tiles (126 VGPRs, four waves per SIMD); 6-row tiles
+// need more than 128 VGPRs and spill. Blocks of 2 or 8 waves and z-chunks of 8 or 32 slices
+// were 1-15 % slower than 4 waves and 16 slices.
+constexpr int kPairTY = 4;
+constexpr int kPairWY = 4;
+
+template <typename S>
+__global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
+    Fused2Ctx<S, CoefF, float> p) {
+#pragma clang fp contract(off)
+  typedef float C;
+  constexpr int TY = kPairTY, WY = kPairWY;
+  constexpr int NA = TY + 4;  // P^n rows (2-voxel y halo)
+  constexpr int NQ = TY + 2;  // P^{n+1} rows (1-voxel y halo)
+  __shared__ int s_src[WY][kFListCap];
+  __shared__ int s_rec[WY][kFListCap];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const Fused2Args<S, CoefF, float> &a = p.a;
+  const int X = p.X, Y = p.Y, Z = p.Z, zc = p.zc;
+  // XCD-aware tile order, as in srl_update2_kernel
+  const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+  const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned full = total & ~7u;
+  const unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+  const int xt = (int)(tile % nx);
+  const int byi = (int)((tile / nx) % ny);
+  const int bzi = (int)(tile / (nx * ny));
+  const int xb = xt * kPO;               // first owned column
+  const int x0 = xb - 2 + 2 * lane;      // this lane's columns: x0 and x0 + 1
+  const int yt = (byi * WY + wave) * TY;  // first owned row
+  const int z0 = 1 + bzi * zc;
+  const int z1 = min(z0 + zc, Z - 1);
+
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
+  const unsigned ncol = (unsigned)a.ncol;
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WY)
+    s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
+  const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kPO + 1, yt - 1, yt + TY + 1,
+                                z0 - 1, z1 + 1, s_src[wave], lane);
+  const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kPO, yt, yt + TY, z0, z1,
+                                s_rec[wave], lane);
+  __syncthreads();
+
+  const size_t XY = (size_t)X * (size_t)Y;
+  const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^32
+  const int xc = min(max(x0, 0), X - 2);  // an even column of the grid
+  uint32_t ro[NA];
+  uint32_t row_upd = 0;
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    const int y = yt - 2 + r;
+    ro[r] = (uint32_t)(min(max(y, 0), Y - 1) * X + xc);
+    if (y > 0 && y < Y - 1) row_upd |= 1u << r;
+  }
+  const bool cu0 = x0 > 0 && x0 < X - 1;
+  const bool cu1 = x0 + 1 > 0 && x0 + 1 < X - 1;
+  const bool owned_lane = lane >= 1 && lane <= 62 && x0 < X;
+  uint32_t row_st = 0;
+#pragma unroll
+  for (int o = 0; o < TY; ++o)
+    if (yt + o < Y) row_st |= 1u << o;
+  const double inv_scale = ldexp(1.0, -a.scale_exp);
+  const C scale = (C)ldexp(1.0, a.scale_exp);
+  const CoefF air = a.air;
+  const C lam2 = a.lam2;
+  // Plain air with the entry (inv, omb, tk) = (1, 1, 0) -- every air voxel when lambda^2 rounds
+  // to 1/3 in FP32, since 2 - 6 lambda^2 is then exactly 0 -- takes
+  //   fma(0, p, lambda^2 s) - p_past,
+  // the same bits as inv*((tk*p + lambda^2 s) - omb*p_past): the products by 1 are exact, and the
+  // product 0*p is exact, so the fused add rounds exactly as the separate one (signed zeros
+  // included). Otherwise every slice takes the table.
+  const bool air_fast = a.air_uniform != 0 && air.inv == 1.f && air.omb == 1.f && air.tk == 0.f;
+  auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
+    if constexpr (decltype(table)::value) {
+      const unsigned pb = pm & 0xFFu;
+      const CoefF e = s_coef[min(pb & 0x7Fu, 6u) * ncol + (pm >> 8)];
+      const C aa = (C)(pb >> 7) * e.inv;
+      return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
+    } else {
+      return __builtin_fmaf(0.f, pc, lam2 * s) - past;
+    }
+  };
+  auto any_nonair = [&](const uint32_t (&b)[NQ]) -> bool {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) m |= ((b[i] & 0xFFu) ^ 0x86u) | (((b[i] >> 16) & 0xFFu) ^ 0x86u);
+    return __ballot(m != 0u) != 0;
+  };
+  const S *__restrict__ gA = a.A;
+  const S *__restrict__ gB = a.B;
+  S *__restrict__ gC = a.Cn;
+  S *__restrict__ gD = a.Dn;
+  const uint16_t *__restrict__ gpm = a.pm;
+  typedef const __attribute__((address_space(4))) uint32_t *ConstU32;
+  const uint32_t *__restrict__ gar = a.air_rows;
+  const int nyw = Y / 32 + 2;
+  const size_t ar_slice = (size_t)((X + kPO - 1) / kPO) * (size_t)nyw;
+  const bool past_grid = yt >= Y || xb >= X;
+  const uint32_t *ar_tile = gar && !past_grid ? gar + (size_t)xt * nyw + (yt >> 5) : nullptr;
+  auto air_window = [&](int k) -> uint64_t {
+    if (!gar || k < 1 || k > Z - 2) return 0;
+    if (past_grid) return ~0ull;
+    const ConstU32 w = (ConstU32)(ar_tile + (size_t)k * ar_slice);
+    return (((uint64_t)w[1] << 32) | w[0]) >> (yt & 31);
+  };
+  constexpr uint64_t kAllRows = (1ull << NQ) - 1ull;
+  uint64_t ar_win = air_window(z0 - 1);
+
+  // P^n rows as (column x0, column x0+1) pairs of FP32 values, P^{n+1} likewise, packed
+  // position|material of both columns in one register per row.
+  // The first level at slice k reads A[k+1], B[k] and the packed plane at k: fetch() issues
+  // those loads into raw 32-bit registers (pairs of 16-bit values). Issuing slice k+1's loads
+  // one body ahead (20 more VGPRs, three waves per SIMD) was 2 % slower.
+  auto fetch = [&](int k, uint32_t (&ua)[NA], uint32_t (&ub)[NQ], uint32_t (&um)[NQ],
+                   bool &uair, uint64_t win) {
+    const bool upd = k >= 1 && k <= Z - 2;
+    if (k + 1 <= Z - 1) {
+      const auto rA = slice_rsrc(gA + (size_t)(k + 1) * XY, sbytes);
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ua[r] = bld_pair(rA, ro[r]);
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ua[r] = 0;
+    }
+    uair = upd && (win & kAllRows) == kAllRows;
+    if (upd) {
+      const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) ub[j] = bld_pair(rB, ro[j + 1]);
+      if (uair) {
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) um[j] = 0x00860086u;
+      } else {
+        const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) um[j] = bld_pair(rPM, ro[j + 1]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        ub[j] = 0;
+        um[j] = 0;
+      }
+    }
+  };
+  auto body = [&](Pair2 (&am)[NA], Pair2 (&ac)[NA], Pair2 (&ap)[NA], Pair2 (&q0)[NQ],
+                  Pair2 (&q1)[NQ], Pair2 (&q2)[NQ], uint32_t (&bp)[NQ], uint32_t (&bc)[NQ],
+                  int k) {
+    uint32_t ru = __builtin_amdgcn_readfirstlane(row_upd);
+    uint32_t rs = __builtin_amdgcn_readfirstlane(row_st);
+    asm volatile("" : "+s"(ru), "+s"(rs));
+    __builtin_amdgcn_s_barrier();
+    const bool upd1 = k >= 1 && k <= Z - 2;
+    uint32_t ua[NA], pu[NQ];
+    bool air1;
+    fetch(k, ua, pu, bc, air1, ar_win);  // ar_win: the air-row window of slice k
+    ar_win = air_window(k + 1);
+    // every load of the slice is in flight before any result is combined
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) ap[r] = Pair2{lo_half<S>(ua[r]), hi_half<S>(ua[r])};
+
+    // ---- first time level: P^{n+1}[k]
+    auto level1 = [&](auto table) {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const int r = j + 1;
+        C s0 = ap[r].x + am[r].x;            // z+1, z-1
+        s0 = s0 + ac[r + 1].x;               // y+1
+        s0 = s0 + ac[r - 1].x;               // y-1
+        s0 = s0 + ac[r].y;                   // x+1: the pair's other column
+        s0 = s0 + lane_prev(ac[r].y);        // x-1: the previous lane's upper column
+        C s1 = ap[r].y + am[r].y;
+        s1 = s1 + ac[r + 1].y;
+        s1 = s1 + ac[r - 1].y;
+        s1 = s1 + lane_next(ac[r].x);        // x+1: the next lane's lower column
+        s1 = s1 + ac[r].x;                   // x-1
+        const C v0 = Fmt<S>::round(node(table, bc[j] & 0xFFFFu, ac[r].x, s0, lo_half<S>(pu[j])));
+        const C v1 = Fmt<S>::round(node(table, bc[j] >> 16, ac[r].y, s1, hi_half<S>(pu[j])));
+        const bool row = upd1 && ((ru >> r) & 1u);
+        q2[j].x = (row && cu0) ? v0 : (C)0;
+        q2[j].y = (row && cu1) ? v1 : (C)0;
+      }
+    };
+    if (!air_fast || (!air1 && any_nonair(bc))) {
+      level1(std::true_type());
+    } else {
+      level1(std::false_type());
+    }
+
+    // ---- receivers (before injection) and sources on P^{n+1}[k]
+    if (nrec != 0) {
+      const int cnt = nrec < 0 ? a.n_rec : nrec;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nrec < 0 ? t : s_rec[wave][t];
+        const int32_t *q = a.rec_xyz + 3 * id;
+        if (q[2] != k || (q[0] & ~1) != x0) continue;
+        if (nrec < 0 && !in_box(q, xb, xb + kPO, yt, yt + TY, z0, z1)) continue;
+        const bool hi = q[0] & 1;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          if (q[1] == yt - 1 + j)
+            a.resp[(int64_t)id * a.resp_steps + a.gather_step] =
+                (double)pick(q2[j], hi) * inv_scale;
+      }
+    }
+    if (nsrc != 0 && a.inject_step >= 0) {
+      const int cnt = nsrc < 0 ? a.n_src : nsrc;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nsrc < 0 ? t : s_src[wave][t];
+        const int32_t *q = a.src_xyz + 3 * id;
+        if (q[2] != k || (q[0] & ~1) != x0) continue;
+        C smp = (C)0;
+        if (a.inject_step < a.n_samples) smp = a.sig_f[(int64_t)id * a.n_samples + a.inject_step];
+        smp = smp * scale;
+        const bool hard = a.src_kind[id] == 0 || a.soft_overwrite;
+        const bool hi = q[0] & 1;
+        // value selects only: a reference chosen at run time would put the queue in memory
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          if (q[1] == yt - 1 + j) {
+            const C v = hard ? Fmt<S>::round(smp) : Fmt<S>::round(pick(q2[j], hi) + smp);
+            q2[j].x = hi ? q2[j].x : v;
+            q2[j].y = hi ? v : q2[j].y;
+          }
+      }
+    }
+
+    // ---- store the owned part of P^{n+1}[k]
+    if (k >= z0 && k < z1) {
+      const auto rC = slice_rsrc(gC + (size_t)k * XY, sbytes);
+#pragma unroll
+      for (int o = 0; o < TY; ++o)
+        if (((rs >> o) & 1u) && owned_lane)
+          bst_pair(rC, ro[o + 2], pack_halves<S>(q2[o + 1].x, q2[o + 1].y));
+    }
+
+    // ---- second time level: P^{n+2}[k-1] on the owned rows
+    if (k - 1 >= z0) {
+      const auto rD = slice_rsrc(gD + (size_t)(k - 1) * XY, sbytes);
+      auto level2 = [&](auto table) {
+#pragma unroll
+        for (int o = 0; o < TY; ++o) {
+          const int j = o + 1;
+          C s0 = q2[j].x + q0[j].x;          // z+1, z-1
+          s0 = s0 + q1[j + 1].x;             // y+1
+          s0 = s0 + q1[j - 1].x;             // y-1
+          s0 = s0 + q1[j].y;                 // x+1
+          s0 = s0 + lane_prev(q1[j].y);      // x-1
+          C s1 = q2[j].y + q0[j].y;
+          s1 = s1 + q1[j + 1].y;
+          s1 = s1 + q1[j - 1].y;
+          s1 = s1 + lane_next(q1[j].x);
+          s1 = s1 + q1[j].x;
+          C v0 = node(table, bp[j] & 0xFFFFu, q1[j].x, s0, am[o + 2].x);
+          C v1 = node(table, bp[j] >> 16, q1[j].y, s1, am[o + 2].y);
+          const bool row = (ru >> (o + 2)) & 1u;
+          v0 = (row && cu0) ? v0 : (C)0;
+          v1 = (row && cu1) ? v1 : (C)0;
+          if (((rs >> o) & 1u) && owned_lane) bst_pair(rD, ro[o + 2], pack_halves<S>(v0, v1));
+        }
+      };
+      if (!air_fast || any_nonair(bp)) {
+        level2(std::true_type());
+      } else {
+        level2(std::false_type());
+      }
+    }
+  };
+
+  Pair2 A0[NA], A1[NA], A2[NA];
+  Pair2 Q0[NQ], Q1[NQ], Q2[NQ];
+  uint32_t B0[NQ], B1[NQ], B2[NQ];
+  const int k0 = z0 - 1;
+  const int k1 = z1;
+  {
+    const auto rK = slice_rsrc(gA + (size_t)k0 * XY, sbytes);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      const uint32_t u = bld_pair(rK, ro[r]);
+      A1[r] = Pair2{lo_half<S>(u), hi_half<S>(u)};
+    }
+    if (k0 >= 1) {
+      const auto rKm = slice_rsrc(gA + (size_t)(k0 - 1) * XY, sbytes);
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const uint32_t u = bld_pair(rKm, ro[r]);
+        A0[r] = Pair2{lo_half<S>(u), hi_half<S>(u)};
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) A0[r] = Pair2{0.f, 0.f};
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      Q0[j] = Pair2{0.f, 0.f};
+      Q1[j] = Pair2{0.f, 0.f};
+      B0[j] = 0;
+    }
+  }
+  for (int k = k0;;) {
+    body(A0, A1, A2, Q0, Q1, Q2, B0, B1, k);
+    if (++k > k1) break;
+    body(A1, A2, A0, Q1, Q2, Q0, B1, B2, k);
+    if (++k > k1) break;
+    body(A2, A0, A1, Q2, Q0, Q1, B2, B0, k);
+    if (++k > k1) break;
+  }
+}
+
 // Air-row plane: bit b of word (z * ntx + t) * nyw + w is set when every voxel of row
 // y = 32 w + b - 1 of slice z in columns [60 t - 1, 60 t + 61) -- the columns whose first-level
 // results wave tile t uses -- is plain air (packed value `air`: position 0x86 for the forward
 // scheme, 0x80 for the centred one; material 0); rows outside the grid are set. One wave per
 // word, lanes over the columns; blocks stride over words and slices.
+// The same for the tiles of the two-column sweep (tile width kPO, window [124 t - 1, 124 t + 125)).
 __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict__ pm,
                                                       uint32_t *__restrict__ out, int X, int Y,
-                                                      int Z, int ntx, int nyw, unsigned air) {
+                                                      int Z, int ntx, int nyw, unsigned air,
+                                                      int tw) {
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
-  const int x = t * kFO - 1 + lane;
-  const bool col = lane < kFO + 2 && x >= 0 && x < X;
   for (int z = blockIdx.z; z < Z; z += gridDim.z) {
     const uint16_t *slice = pm + (size_t)z * (size_t)X * (size_t)Y;
     for (int w = blockIdx.y; w < nyw; w += gridDim.y) {
       uint32_t word = 0;
       for (int b = 0; b < 32; ++b) {
         const int y = 32 * w + b - 1;
-        const bool other = col && y >= 0 && y < Y && slice[(size_t)y * X + x] != air;
+        bool other = false;
+        for (int c = lane; c < tw + 2; c += 64) {
+          const int x = t * tw - 1 + c;
+          other = other || (x >= 0 && x < X && y >= 0 && y < Y && slice[(size_t)y * X + x] != air);
+        }
         if (__ballot(other) == 0) word |= 1u << b;
       }
       if (lane == 0) out[((size_t)z * ntx + t) * nyw + w] = word;
@@ -599,16 +961,19 @@ __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict
   }
 }
 
-size_t air_rows_words(int X, int Y, int Z) {
-  return (size_t)((X + kFO - 1) / kFO) * (size_t)(Y / 32 + 2) * (size_t)Z;
+size_t air_rows_words(int X, int Y, int Z, bool pair) {
+  const int tw = pair ? kPO : kFO;
+  return (size_t)((X + tw - 1) / tw) * (size_t)(Y / 32 + 2) * (size_t)Z;
 }
 
 hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z, int scheme,
-                           hipStream_t stream) {
-  const int ntx = (X + kFO - 1) / kFO, nyw = Y / 32 + 2;
+                           bool pair, hipStream_t stream) {
+  const int tw = pair ? kPO : kFO;
+  const int ntx = (X + tw - 1) / tw, nyw = Y / 32 + 2;
   const dim3 grid((unsigned)ntx, (unsigned)std::min(nyw, 4096), (unsigned)std::min(Z, 4096));
   const unsigned air = scheme == 2 ? 0x0080u : 0x0086u;
-  hipLaunchKernelGGL(air_rows_kernel, grid, dim3(64), 0, stream, pm, rows, X, Y, Z, ntx, nyw, air);
+  hipLaunchKernelGGL(air_rows_kernel, grid, dim3(64), 0, stream, pm, rows, X, Y, Z, ntx, nyw, air,
+                     tw);
   return hipGetLastError();
 }
 
@@ -626,6 +991,22 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   g.scheme = scheme;
   g.wide = !pool && precision == 0 && scheme == 0 &&
            (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16));
+  // 16-bit storage, forward scheme, X even: two columns per lane
+  g.pair = !pool && precision >= 2 && scheme == 0 && X % 2 == 0 &&
+           (shape == PFDTD_SWEEP_AUTO || shape == PFDTD_SWEEP_PAIR);
+  if (g.pair) {
+    const int rows = kPairTY * kPairWY;
+    const int ptx = (X + kPO - 1) / kPO;
+    const int64_t pxy = (int64_t)ptx * ((Y + rows - 1) / rows);
+    if (zc < 1) {
+      zc = 16;
+      while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
+    }
+    g.zc = zc > zi ? zi : zc;
+    g.grid = dim3(ptx, (Y + rows - 1) / rows, (zi + g.zc - 1) / g.zc);
+    g.block = dim3(64 * kPairWY, 1, 1);
+    return g;
+  }
   const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
   const int WY = g.wide ? FusedShape<true>::WY : FusedShape<false>::WY;
   const int64_t bxy = (int64_t)((ntx + WX - 1) / WX) * ((Y + kFTY * WY - 1) / (kFTY * WY));
@@ -648,6 +1029,12 @@ static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a
                                hipStream_t stream) {
   const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
   const Fused2Ctx<S, CT, C> p{a, g.X, g.Y, g.Z, g.zc};
+  if constexpr (sizeof(S) == 2) {
+    if (g.pair) {
+      hipLaunchKernelGGL((srl_update2_pair_kernel<S>), g.grid, g.block, lds, stream, p);
+      return hipGetLastError();
+    }
+  }
   if (g.scheme == 2) {
     hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false, 2>), g.grid, g.block, lds,
                        stream, p);

@@ -188,6 +188,7 @@ struct FusedGeom {
   int X, Y, Z;
   int zc;
   bool wide;   // 4 x 4-wave blocks (FP32, forward scheme), else 1 x 4
+  bool pair;   // 16-bit storage: two columns per lane (124-column wave tiles, X even)
   int scheme;  // 0 forward boundaries, 2 centred
   dim3 grid, block;
 };
@@ -198,9 +199,10 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = fal
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.
-size_t air_rows_words(int X, int Y, int Z);
+// `pair`: the tiles of the two-column sweep (124 columns) instead of 60.
+size_t air_rows_words(int X, int Y, int Z, bool pair = false);
 hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z, int scheme,
-                           hipStream_t stream);
+                           bool pair, hipStream_t stream);
 hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
                               hipStream_t stream);
 hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,

@@ -737,7 +737,7 @@ int pfdtd_pool_submit(pfdtd_pool *p, const pfdtd_job *j, int64_t *job_id) {
   if (e == hipSuccess && fused)
     e = launch_pack_pm(d.pos, d.mat, (uint16_t *)(base + L.pm), N, u);
   if (e == hipSuccess && fused)
-    e = launch_air_rows(d.pm, (uint32_t *)(base + L.air_rows), X, Y, Z, 0, u);
+    e = launch_air_rows(d.pm, (uint32_t *)(base + L.air_rows), X, Y, Z, 0, false, u);
   if (e == hipSuccess && fused && j->n_src)
     e = hipMemcpyAsync(base + L.src_xyz, j->src_xyz, sizeof(int32_t) * 3 * j->n_src, hipMemcpyHostToDevice, u);
   if (e == hipSuccess && fused && j->n_rec)

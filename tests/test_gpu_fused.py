@@ -215,7 +215,52 @@ def test_two_level_wide_room_automatic_shape(gpu):
 def test_sweep_shape_rejects_unknown(gpu):
     with Solver() as s:
         with pytest.raises(N.PfdtdError):
-            s.set_sweep_shape(3)
+            s.set_sweep_shape(4)
+        with pytest.raises(N.PfdtdError):
+            s.set_sweep_shape(-1)
+
+
+def pair_border_scene(seed, dims, n_steps=30):
+    """Single solid voxels next to the 124-column borders of the two-column sweep's wave tiles
+    (and its 4- and 6-row borders), and sources and receivers on both columns of the pairs either
+    side of those borders (lanes 0, 1, 62 and 63 of a wave) and on the room's last columns."""
+    sc = rooms.scene_random_small(seed, dims=dims, n_mat=3, n_boxes=0, n_src=1, n_rec=1,
+                                  n_steps=n_steps)
+    X, Y, Z = dims
+    borders = {124 * t + d for t in range(1, X // 124 + 2) for d in (-3, -2, -1, 0, 1, 2)}
+    xs = sorted((borders | {2, 3, X - 4, X - 3}) & set(range(2, X - 2)))
+    ys = sorted({m + d for ty in (4, 6) for m in range(ty, Y, ty) for d in (-2, -1, 0, 1)}
+                & set(range(2, Y - 2)))
+    k = 0
+    for x in xs:
+        for y in ys[::2]:
+            z = 2 + (k * 5) % max(1, Z - 4)
+            sc.labels[z, y, x] = 2 + k % 2
+            k += 1
+    pts = np.array([[xs[i % len(xs)], ys[(7 * i) % len(ys)], 1 + (3 * i) % (Z - 2)]
+                    for i in range(2 * len(xs))], np.int32)
+    rng = np.random.default_rng(seed)
+    n_src = min(len(pts), 40)
+    sc.src_xyz = pts[:n_src].copy()
+    sc.src_kind = np.array([i % 2 for i in range(n_src)], np.int32)
+    sc.signals = rng.uniform(-1, 1, (n_src, n_steps))
+    sc.rec_xyz = pts[::-1].copy()
+    return sc
+
+
+@pytest.mark.parametrize("dims", [(250, 40, 18), (372, 27, 14), (126, 25, 12), (8, 9, 7),
+                                  (131, 22, 10)])
+@pytest.mark.parametrize("shape", [N.SWEEP_AUTO, N.SWEEP_PAIR])
+@pytest.mark.parametrize("prec", [N.F16, N.BF16])
+def test_two_level_pair_sweep(gpu, dims, shape, prec):
+    """The two-column 16-bit sweep (automatic on rooms of even X) on rooms of two tiles and a
+    pair, exactly three tiles, one tile and a pair, less than one tile, and odd X (where PAIR
+    falls back to one column per lane): bit-exact against the oracle, boundary voxels, sources
+    and receivers on the pair and tile borders."""
+    sc = pair_border_scene(95, dims)
+    got, ref, F, F_ref = run_both(sc, prec, zchunk=5, shape=shape)
+    assert np.abs(ref).max() > 0
+    assert_same(got, ref, F, F_ref)
 
 
 @pytest.mark.parametrize("prec", [N.F32, N.F16])
@@ -245,3 +290,27 @@ def test_two_level_degenerate_shapes(gpu, dims, prec):
         got, ref, F, F_ref = run_both(sc, prec, zchunk=0, shape=shape)
         assert np.abs(ref).max() > 0
         assert_same(got, ref, F, F_ref)
+
+
+@pytest.mark.parametrize("lam", [0.5, 0.57735])
+def test_two_level_pair_sweep_other_lambda(gpu, lam):
+    """With lambda^2 not rounding to 1/3 the air entry is not (1, 1, 0): the two-column sweep
+    then updates every slice through the coefficient table; bit-exact against the oracle."""
+    sc = pair_border_scene(96, (250, 30, 12), n_steps=24)
+    pos, mat, _ = oracle.classify(sc.labels)
+    sig = sc.signals.astype(np.float32)
+    ref, P_ref, _ = oracle.run(pos, mat, sc.material_table(), sc.src_xyz, sc.src_kind, sig,
+                               sc.rec_xyz, sc.n_steps, lam=lam, storage=oracle.STORE_F16,
+                               scale_exp=9, want_fields=True)
+    with Solver() as s:
+        s.set_domain_labels(sc.labels)
+        s.set_materials(sc.material_table(), lam=lam)
+        s.set_sources(sc.src_xyz, sc.src_kind, signal_f64=sc.signals, signal_f32=sig)
+        s.set_receivers(sc.rec_xyz)
+        s.prepare_fields(N.F16, 9)
+        s.run(sc.n_steps, N.F16)
+        got = s.responses(np.float32)
+        P, _ = s.fields_raw(np.float16)
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(got, ref)
+    assert np.array_equal(P.astype(np.float32), P_ref)
