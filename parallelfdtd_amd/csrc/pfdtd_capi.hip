@@ -458,11 +458,19 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
   return PFDTD_OK;
 }
 
+// Zero current and past fields. Of the spare pair only the first and last z-slices are cleared:
+// the two-level update writes every voxel of slices 1..Z-2 of both spare buffers before they
+// are read (x/y edge voxels as 0), and never writes slices 0 and Z-1, which the next updates
+// read as z-neighbours once the pairs are swapped.
 int zero_fields(pfdtd_ctx *c) {
   HIP_TRY(hipMemsetAsync(c->d_P, 0, c->field_bytes, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_Pp, 0, c->field_bytes, c->stream));
-  if (c->d_P2) HIP_TRY(hipMemsetAsync(c->d_P2, 0, c->field_bytes, c->stream));
-  if (c->d_Pp2) HIP_TRY(hipMemsetAsync(c->d_Pp2, 0, c->field_bytes, c->stream));
+  const size_t slice = c->field_bytes / (size_t)c->Z;
+  for (void *q : {c->d_P2, c->d_Pp2}) {
+    if (!q) continue;
+    HIP_TRY(hipMemsetAsync(q, 0, slice, c->stream));
+    HIP_TRY(hipMemsetAsync((char *)q + (size_t)(c->Z - 1) * slice, 0, slice, c->stream));
+  }
   c->last_direction = 1;
   return PFDTD_OK;
 }
