@@ -141,7 +141,7 @@ def classify_kow(labels: np.ndarray):
     return pos, mat, counts
 
 
-STORE_F32, STORE_F16, STORE_BF16 = 0, 1, 2
+STORE_F32, STORE_F16, STORE_BF16, STORE_F16_DIFF, STORE_F16_SUMDIFF = 0, 1, 2, 3, 4
 SCHEME_KOWALCZYK = 4  # bit of `literal`: update type SRL (centred boundary)
 
 

@@ -460,6 +460,78 @@ static void sweep_f(const uint8_t *pos, const uint8_t *mat, const float *P, floa
   }
 }
 
+/* The same sweep in the difference form (ORC_STORE_F16_DIFF): A = P^n and B = D^n in, C = P^{n+1}
+ * out, B overwritten with D^{n+1}. Each node reads only its own B, so B is updated in place. */
+static void sweep_diff_f(const uint8_t *pos, const uint8_t *mat, const float *A, float *B,
+                         float *C, const float *beta_of_mat, float lam, float lam2, int64_t X,
+                         int64_t Y, int64_t Z, int threads) {
+  const int64_t XY = X * Y;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 1; z <= Z - 2; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      const int64_t row = z * XY + y * X;
+      if (y == 0 || y == Y - 1) {
+        for (int64_t x = 0; x < X; ++x) C[row + x] = B[row + x] = 0.f;
+        continue;
+      }
+      C[row] = B[row] = 0.f;
+      C[row + X - 1] = B[row + X - 1] = 0.f;
+      for (int64_t x = 1; x <= X - 2; ++x) {
+        const int64_t i = row + x;
+        const uint8_t pb = pos[i];
+        const float position = (float)(pb & ORC_POS_MASK);
+        const float switch_bit = (float)(pb >> ORC_INSIDE_SHIFT);
+        const float beta = (float)(0.5 * (double)(beta_of_mat[mat[i]] * (6.f - position) * lam));
+        const float p = A[i];
+        const float p_past = p - B[i];
+        const float S = A[i + XY] + A[i - XY] + A[i + X] + A[i - X] + A[i + 1] + A[i - 1];
+        const float ret = switch_bit * (1.f / (1.f + beta)) *
+                          ((2.f - position * lam2) * p + lam2 * S - (1.f - beta) * p_past);
+        C[i] = store_round(ret, ORC_STORE_F16);
+        B[i] = store_round(ret - p, ORC_STORE_F16);
+      }
+    }
+  }
+}
+
+/* The sweep in the sum-difference form (ORC_STORE_F16_SUMDIFF): Pc = P^n and Pp = P^{n-1}
+ * reconstructed from (S, D) before the sweep; S and D are overwritten with the new state. */
+static void sweep_sumdiff_f(const uint8_t *pos, const uint8_t *mat, const float *Pc,
+                            const float *Pp, float *S, float *D, const float *beta_of_mat,
+                            float lam, float lam2, int64_t X, int64_t Y, int64_t Z,
+                            int threads) {
+  const int64_t XY = X * Y;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+  for (int64_t z = 1; z <= Z - 2; ++z) {
+    for (int64_t y = 0; y < Y; ++y) {
+      const int64_t row = z * XY + y * X;
+      if (y == 0 || y == Y - 1) {
+        for (int64_t x = 0; x < X; ++x) S[row + x] = D[row + x] = 0.f;
+        continue;
+      }
+      S[row] = D[row] = 0.f;
+      S[row + X - 1] = D[row + X - 1] = 0.f;
+      for (int64_t x = 1; x <= X - 2; ++x) {
+        const int64_t i = row + x;
+        const uint8_t pb = pos[i];
+        const float position = (float)(pb & ORC_POS_MASK);
+        const float switch_bit = (float)(pb >> ORC_INSIDE_SHIFT);
+        const float beta = (float)(0.5 * (double)(beta_of_mat[mat[i]] * (6.f - position) * lam));
+        const float p = Pc[i];
+        const float sum = Pc[i + XY] + Pc[i - XY] + Pc[i + X] + Pc[i - X] + Pc[i + 1] + Pc[i - 1];
+        const float ret = switch_bit * (1.f / (1.f + beta)) *
+                          ((2.f - position * lam2) * p + lam2 * sum - (1.f - beta) * Pp[i]);
+        S[i] = store_round(ret + p, ORC_STORE_F16);
+        D[i] = store_round(ret - p, ORC_STORE_F16);
+      }
+    }
+  }
+}
+
 /* fdtd3dStdMaterials<double> sweep (kernels3d.cu:485-529). */
 static void sweep_d(const uint8_t *pos, const uint8_t *mat, const double *P, double *Pp,
                     const double *beta_of_mat, double lam, double lam2, int64_t X, int64_t Y,
@@ -591,6 +663,116 @@ static void sweep_kow_d(const uint8_t *pos, const uint8_t *mat, const double *P,
   }
 }
 
+/* Step loop of the difference form: the source loop, the sweep and the receiver loop of orc_run_f
+ * (kernels3d.cu:83-191) on the state (P, D). */
+static int run_diff_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+                      const float *beta_of_mat, const float params[4], const int32_t *src_xyz,
+                      const int32_t *src_type, const float *signals, int n_src,
+                      const int32_t *rec_xyz, int n_rec, int n_steps, int literal, int scale_exp,
+                      int threads, float *responses, float *P_out, float *D_out) {
+  const int64_t N = X * Y * Z;
+  float *A = (float *)calloc((size_t)N, sizeof(float));
+  float *B = (float *)calloc((size_t)N, sizeof(float));
+  float *Cn = (float *)calloc((size_t)N, sizeof(float));
+  if (!A || !B || !Cn) {
+    free(A);
+    free(B);
+    free(Cn);
+    return -1;
+  }
+  const float scale = ldexpf(1.f, scale_exp);
+  const float inv_scale = ldexpf(1.f, -scale_exp);
+  if (responses) memset(responses, 0, sizeof(float) * (size_t)n_rec * (size_t)n_steps);
+  for (int n = 0; n < n_steps; ++n) {
+    for (int s = 0; s < n_src; ++s) {
+      const int32_t *q = src_xyz + 3 * s;
+      if (!in_grid(q, X, Y, Z)) continue;
+      const int64_t i = lin(q, X, Y);
+      const float smp = signals[(size_t)s * (size_t)n_steps + (size_t)n] * scale;
+      const float past = A[i] - B[i];
+      if (src_type[s] == ORC_SRC_HARD || (literal & 2))
+        A[i] = store_round(smp, ORC_STORE_F16);
+      else
+        A[i] = store_round(A[i] + smp, ORC_STORE_F16);
+      B[i] = store_round(A[i] - past, ORC_STORE_F16);
+    }
+    sweep_diff_f(pos, mat, A, B, Cn, beta_of_mat, params[0], params[1], X, Y, Z, threads);
+    float *t = A;
+    A = Cn;
+    Cn = t;
+    for (int r = 0; r < n_rec; ++r) {
+      const int32_t *q = rec_xyz + 3 * r;
+      if (!in_grid(q, X, Y, Z) || !responses) continue;
+      responses[(size_t)r * (size_t)n_steps + (size_t)n] = A[lin(q, X, Y)] * inv_scale;
+    }
+  }
+  if (P_out) memcpy(P_out, A, sizeof(float) * (size_t)N);
+  if (D_out) memcpy(D_out, B, sizeof(float) * (size_t)N);
+  free(A);
+  free(B);
+  free(Cn);
+  return 0;
+}
+
+/* Step loop of the sum-difference form: the source loop, the sweep and the receiver loop of
+ * orc_run_f (kernels3d.cu:83-191) on the state (S, D). */
+static int run_sumdiff_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
+                         const float *beta_of_mat, const float params[4], const int32_t *src_xyz,
+                         const int32_t *src_type, const float *signals, int n_src,
+                         const int32_t *rec_xyz, int n_rec, int n_steps, int literal,
+                         int scale_exp, int threads, float *responses, float *S_out,
+                         float *D_out) {
+  const int64_t N = X * Y * Z;
+  float *S = (float *)calloc((size_t)N, sizeof(float));
+  float *D = (float *)calloc((size_t)N, sizeof(float));
+  float *Pc = (float *)calloc((size_t)N, sizeof(float));
+  float *Pp = (float *)calloc((size_t)N, sizeof(float));
+  if (!S || !D || !Pc || !Pp) {
+    free(S);
+    free(D);
+    free(Pc);
+    free(Pp);
+    return -1;
+  }
+  const float scale = ldexpf(1.f, scale_exp);
+  const float inv_scale = ldexpf(1.f, -scale_exp);
+  if (responses) memset(responses, 0, sizeof(float) * (size_t)n_rec * (size_t)n_steps);
+  for (int n = 0; n < n_steps; ++n) {
+    for (int s = 0; s < n_src; ++s) {
+      const int32_t *q = src_xyz + 3 * s;
+      if (!in_grid(q, X, Y, Z)) continue;
+      const int64_t i = lin(q, X, Y);
+      const float smp = signals[(size_t)s * (size_t)n_steps + (size_t)n] * scale;
+      const float p = (S[i] + D[i]) * 0.5f;
+      const float past = (S[i] - D[i]) * 0.5f;
+      const float pn = (src_type[s] == ORC_SRC_HARD || (literal & 2)) ? smp : p + smp;
+      S[i] = store_round(pn + past, ORC_STORE_F16);
+      D[i] = store_round(pn - past, ORC_STORE_F16);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(threads) schedule(static)
+#endif
+    for (int64_t i = 0; i < N; ++i) {
+      Pc[i] = (S[i] + D[i]) * 0.5f;
+      Pp[i] = (S[i] - D[i]) * 0.5f;
+    }
+    sweep_sumdiff_f(pos, mat, Pc, Pp, S, D, beta_of_mat, params[0], params[1], X, Y, Z, threads);
+    for (int r = 0; r < n_rec; ++r) {
+      const int32_t *q = rec_xyz + 3 * r;
+      if (!in_grid(q, X, Y, Z) || !responses) continue;
+      const int64_t i = lin(q, X, Y);
+      responses[(size_t)r * (size_t)n_steps + (size_t)n] = (S[i] + D[i]) * 0.5f * inv_scale;
+    }
+  }
+  if (S_out) memcpy(S_out, S, sizeof(float) * (size_t)N);
+  if (D_out) memcpy(D_out, D, sizeof(float) * (size_t)N);
+  free(S);
+  free(D);
+  free(Pc);
+  free(Pp);
+  return 0;
+}
+
 int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int64_t Z,
               const float *mat_table, int n_mat, unsigned octave, double lambda,
               const int32_t *src_xyz, const int32_t *src_type, const float *signals, int n_src,
@@ -598,7 +780,8 @@ int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
               int scale_exp, int threads, float *responses, float *P_out, float *Pp_out) {
   int rc = check_common(pos, mat, X, Y, Z, n_mat, octave, literal, n_src, n_rec, n_steps);
   if (rc) return rc;
-  if (!mat_table || storage < 0 || storage > 2 || scale_exp < -100 || scale_exp > 100) return -1;
+  if (!mat_table || storage < 0 || storage > 4 || scale_exp < -100 || scale_exp > 100) return -1;
+  if (storage >= ORC_STORE_F16_DIFF && (literal & ORC_SCHEME_KOWALCZYK)) return -1;
   if (threads < 1) threads = 1;
   float params[4];
   orc_param_vector_f(lambda, octave, params);
@@ -607,6 +790,14 @@ int orc_run_f(const uint8_t *pos, const uint8_t *mat, int64_t X, int64_t Y, int6
     const int64_t slot = orc_material_slot((unsigned)m, octave, (literal & ORC_SCHEME_KOWALCZYK) ? 0 : (literal & 1));
     beta_of_mat[m] = slot < (int64_t)n_mat * ORC_COEF_NUM ? mat_table[slot] : 0.f;
   }
+  if (storage == ORC_STORE_F16_SUMDIFF)
+    return run_sumdiff_f(pos, mat, X, Y, Z, beta_of_mat, params, src_xyz, src_type, signals,
+                         n_src, rec_xyz, n_rec, n_steps, literal, scale_exp, threads, responses,
+                         P_out, Pp_out);
+  if (storage == ORC_STORE_F16_DIFF)
+    return run_diff_f(pos, mat, X, Y, Z, beta_of_mat, params, src_xyz, src_type, signals, n_src,
+                      rec_xyz, n_rec, n_steps, literal, scale_exp, threads, responses, P_out,
+                      Pp_out);
   const int64_t N = X * Y * Z;
   float *A = (float *)calloc((size_t)N, sizeof(float));
   float *B = (float *)calloc((size_t)N, sizeof(float));

@@ -37,6 +37,18 @@ extern "C" {
 #define ORC_STORE_F32 0
 #define ORC_STORE_F16 1
 #define ORC_STORE_BF16 2
+/* Difference form (no reference counterpart): the stored state is P^n and D^n = P^n - P^{n-1},
+ * both binary16 (scaled). Each update forms the past value as P^n - D^n in FP32, evaluates the
+ * reference expression, and stores P^{n+1} = fl16(ret) and D^{n+1} = fl16(ret - P^n), ret the
+ * FP32 result. A source sample changes P^n and keeps the past value: D^n = fl16(P^n_new - past).
+ * P_out / Pp_out then hold P and D. */
+#define ORC_STORE_F16_DIFF 3
+/* Sum-difference form (no reference counterpart): the stored state is S^n = P^n + P^{n-1} and
+ * D^n = P^n - P^{n-1}, both binary16 (scaled). Each update reconstructs P^n = (S^n + D^n)/2 and the
+ * past value (S^n - D^n)/2 in FP32, evaluates the reference expression, and stores
+ * S^{n+1} = fl16(ret + P^n), D^{n+1} = fl16(ret - P^n). A source sample changes P^n and keeps the
+ * past value. Receivers read the reconstructed P. P_out / Pp_out then hold S and D. */
+#define ORC_STORE_F16_SUMDIFF 4
 
 /* Bit 2 of the `literal` argument of orc_run_*: update type SRL (centred Kowalczyk boundary,
  * enum UpdateType, src/base/SimulationParameters.h:39) instead of SRL_FORWARD. */

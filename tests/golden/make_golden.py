@@ -84,6 +84,102 @@ def run(pos, mat, betas, octave, lam, src, kinds, sig, rec, n_steps, dt, literal
     return resp, P, Pp
 
 
+def run_f16(pos, mat, betas, lam, src, kinds, sig, rec, n_steps, scale_exp, literal=0,
+            diff=False):
+    """FP32 arithmetic with binary16 field storage scaled by 2^scale_exp: the plain storage of
+    (P, P_past), or (diff) the difference form (P, D = P - P_past): the past value is P - D, each
+    update stores fl16(new) and fl16(new - P), and a source sample keeps the past value,
+    D = fl16(P_new - (P_old - D_old))."""
+    Z, Y, X = pos.shape
+    f = np.float32
+    r16 = lambda v: np.asarray(v, f).astype(np.float16).astype(f)
+    lamv, lam2 = f(lam), f(lam * lam)
+    K = (pos & 0x7F).astype(f)
+    sw = (pos >> 7).astype(f)
+    b = np.asarray(betas, f)[mat]
+    beta = (0.5 * ((b * (f(6) - K)) * lamv).astype(np.float64)).astype(f)
+    a = sw * (f(1) / (f(1) + beta))
+    tk = f(2) - K * lam2
+    omb = f(1) - beta
+    scale, inv_scale = f(2.0 ** scale_exp), f(2.0 ** -scale_exp)
+    P = np.zeros((Z, Y, X), f)
+    B = np.zeros((Z, Y, X), f)  # P_past, or D in the difference form
+    resp = np.zeros((len(rec), n_steps), f)
+    c = (slice(1, Z - 1), slice(1, Y - 1), slice(1, X - 1))
+    for n in range(n_steps):
+        for s, (x, y, z) in enumerate(src):
+            v = f(sig[s, n]) * scale
+            hard = kinds[s] == 0 or (literal & 2)
+            past = P[z, y, x] - B[z, y, x]
+            P[z, y, x] = r16(v) if hard else r16(P[z, y, x] + v)
+            if diff:
+                B[z, y, x] = r16(P[z, y, x] - past)
+        S = P[2:, 1:-1, 1:-1] + P[:-2, 1:-1, 1:-1]
+        S = S + P[1:-1, 2:, 1:-1]
+        S = S + P[1:-1, :-2, 1:-1]
+        S = S + P[1:-1, 1:-1, 2:]
+        S = S + P[1:-1, 1:-1, :-2]
+        past = P[c] - B[c] if diff else B[c]
+        new = a[c] * ((tk[c] * P[c] + lam2 * S) - omb[c] * past)
+        Pn = np.zeros((Z, Y, X), f)  # slices 0 and Z-1 and the x/y edges stay 0
+        Pn[c] = r16(new)
+        if diff:
+            Bn = np.zeros((Z, Y, X), f)
+            Bn[c] = r16(new - P[c])
+            P, B = Pn, Bn
+        else:
+            P, B = Pn, P
+        for r, (x, y, z) in enumerate(rec):
+            resp[r, n] = P[z, y, x] * inv_scale
+    return resp, P, B
+
+
+def run_sumdiff16(pos, mat, betas, lam, src, kinds, sig, rec, n_steps, scale_exp, literal=0):
+    """The sum-difference form: binary16 S = P + P_past and D = P - P_past (scaled), P and P_past
+    reconstructed as (S + D)/2 and (S - D)/2 in FP32; each update stores fl16(new + P) and
+    fl16(new - P); a source sample keeps the past value; receivers read (S + D)/2."""
+    Z, Y, X = pos.shape
+    f = np.float32
+    r16 = lambda v: np.asarray(v, f).astype(np.float16).astype(f)
+    lamv, lam2 = f(lam), f(lam * lam)
+    K = (pos & 0x7F).astype(f)
+    sw = (pos >> 7).astype(f)
+    b = np.asarray(betas, f)[mat]
+    beta = (0.5 * ((b * (f(6) - K)) * lamv).astype(np.float64)).astype(f)
+    a = sw * (f(1) / (f(1) + beta))
+    tk = f(2) - K * lam2
+    omb = f(1) - beta
+    scale, inv_scale, h = f(2.0 ** scale_exp), f(2.0 ** -scale_exp), f(0.5)
+    S = np.zeros((Z, Y, X), f)
+    D = np.zeros((Z, Y, X), f)
+    resp = np.zeros((len(rec), n_steps), f)
+    c = (slice(1, Z - 1), slice(1, Y - 1), slice(1, X - 1))
+    for n in range(n_steps):
+        for s, (x, y, z) in enumerate(src):
+            v = f(sig[s, n]) * scale
+            p = (S[z, y, x] + D[z, y, x]) * h
+            past = (S[z, y, x] - D[z, y, x]) * h
+            pn = v if (kinds[s] == 0 or (literal & 2)) else p + v
+            S[z, y, x], D[z, y, x] = r16(pn + past), r16(pn - past)
+        P = (S + D) * h
+        Pp = (S - D) * h
+        T = P[2:, 1:-1, 1:-1] + P[:-2, 1:-1, 1:-1]
+        T = T + P[1:-1, 2:, 1:-1]
+        T = T + P[1:-1, :-2, 1:-1]
+        T = T + P[1:-1, 1:-1, 2:]
+        T = T + P[1:-1, 1:-1, :-2]
+        new = a[c] * ((tk[c] * P[c] + lam2 * T) - omb[c] * Pp[c])
+        Sn = np.zeros((Z, Y, X), f)
+        Dn = np.zeros((Z, Y, X), f)
+        Sn[0], Sn[-1], Dn[0], Dn[-1] = S[0], S[-1], D[0], D[-1]
+        Sn[c] = r16(new + P[c])
+        Dn[c] = r16(new - P[c])
+        S, D = Sn, Dn
+        for r, (x, y, z) in enumerate(rec):
+            resp[r, n] = (S[z, y, x] + D[z, y, x]) * h * inv_scale
+    return resp, S, D
+
+
 def splitmix(seed):
     state = seed & (2**64 - 1)
     while True:
@@ -122,10 +218,18 @@ def case(seed: int, dims, n_mat: int, n_steps: int, literal: int, octave: int):
                          src, kinds, sig, rec, n_steps, np.float64, literal)
     r32, P32, Pp32 = run(pos, mat, np.array(betas, np.float32), octave, lam, src, kinds,
                          sig.astype(np.float32), rec, n_steps, np.float32, literal)
+    scale = 9
+    rs16, Ps16, Pps16 = run_f16(pos, mat, betas, lam, src, kinds, sig, rec, n_steps, scale, literal)
+    rd16, Pd16, Dd16 = run_f16(pos, mat, betas, lam, src, kinds, sig, rec, n_steps, scale, literal,
+                               diff=True)
+    rsd16, Ssd16, Dsd16 = run_sumdiff16(pos, mat, betas, lam, src, kinds, sig, rec, n_steps, scale,
+                                        literal)
     return dict(labels=lab, table=table, octave=octave, literal=literal, lam=lam,
                 src=np.array(src, np.int32), kinds=np.array(kinds, np.int32), signals=sig,
                 rec=np.array(rec, np.int32), n_steps=n_steps, pos=pos, mat=mat,
-                resp64=r64, P64=P64, Pp64=Pp64, resp32=r32, P32=P32, Pp32=Pp32)
+                resp64=r64, P64=P64, Pp64=Pp64, resp32=r32, P32=P32, Pp32=Pp32,
+                scale16=scale, resp_s16=rs16, P_s16=Ps16, Pp_s16=Pps16, resp_d16=rd16, P_d16=Pd16,
+                D_d16=Dd16, resp_sd16=rsd16, S_sd16=Ssd16, D_sd16=Dsd16)
 
 
 CASES = [
