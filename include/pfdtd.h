@@ -37,6 +37,12 @@ extern "C" {
 #define PFDTD_F64 1  /* FP64 fields and arithmetic: launchFDTD3dDouble */
 #define PFDTD_F16 2  /* FP32 arithmetic, binary16 field storage, power-of-two scaled */
 #define PFDTD_BF16 3 /* FP32 arithmetic, bfloat16 field storage, power-of-two scaled */
+/* FP32 arithmetic, binary16 storage of P^n and D^n = P^n - P^{n-1} (difference form, power-of-two
+ * scaled; no reference counterpart): a storage rounding of P no longer changes the velocity
+ * P^n - P^{n-1}, which cuts the in-band error growth of the scheme's marginal low-frequency
+ * modes. pfdtd_run only, through the two-column sweep: forward update, X even, no batch, no
+ * source on the slices z = 0 or Z-1. pfdtd_get_fields_raw returns P and D. */
+#define PFDTD_F16_DIFF 4
 
 /* ---- source kinds: enum SrcType, src/base/SrcRec.h:33 ----------------------------- */
 #define PFDTD_SRC_HARD 0

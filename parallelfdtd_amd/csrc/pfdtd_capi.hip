@@ -48,11 +48,15 @@ size_t elem_size(int precision) {
     return 8;
   case PFDTD_F16:
   case PFDTD_BF16:
+  case PFDTD_F16_DIFF:
     return 2;
   default:
     return 0;
   }
 }
+
+// Storage format of the current field P (the difference form keeps P in binary16).
+int p_storage(int precision) { return precision == PFDTD_F16_DIFF ? PFDTD_F16 : precision; }
 
 template <typename T>
 void dfree(T *&p) {
@@ -579,7 +583,9 @@ Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C la
   a.sig_f = c->d_sig_f;
   a.sig_d = c->d_sig_d;
   a.n_samples = c->n_samples;
-  a.inject_step = c->n_src > 0 ? step + 1 : -1;
+  a.inject_step = (c->n_src > 0 && step + 1 < c->resp_steps) ? step + 1 : -1;
+  a.levels = 2;
+  a.diff = c->precision == PFDTD_F16_DIFF ? 1 : 0;
   a.soft_overwrite = (c->flags & PFDTD_SOFT_OVERWRITE) ? 1 : 0;
   a.scale_exp = c->scale_exp;
   a.rec_xyz = c->d_rec_xyz;
@@ -590,8 +596,14 @@ Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C la
   return a;
 }
 
-// Steps `step` and `step+1` in one sweep: P^{step+1} lands in d_Pp2, P^{step+2} in d_P2.
-hipError_t launch_update2_any(pfdtd_ctx *c, const FusedGeom &g, int step) {
+// Steps `step` and `step+1` in one sweep: P^{step+1} lands in d_Pp2, P^{step+2} in d_P2. The
+// difference form stores P in d_P2 and D in d_Pp2 after `levels` steps (2, or 1 for its last).
+hipError_t launch_update2_any(pfdtd_ctx *c, const FusedGeom &g, int step, int levels = 2) {
+  if (c->precision == PFDTD_F16_DIFF) {
+    auto a = fused_args<h16>(c, c->d_coef_f, c->air_f, c->lam2_f, step, g.pair);
+    a.levels = levels;
+    return launch_update2_f16(g, a, c->stream);
+  }
   switch (c->precision) {
   case PFDTD_F32:
     return launch_update2_f32(
@@ -1041,7 +1053,8 @@ int pfdtd_prepare_fields(pfdtd_ctx *c, int precision, int scale_exp) {
   if (!c->has_domain) return fail(PFDTD_E_STATE, "no domain set");
   rc = ensure_fields(c, precision, scale_exp);
   if (rc) return rc;
-  if (precision == PFDTD_F16 || precision == PFDTD_BF16) c->scale_req = scale_exp;
+  if (precision == PFDTD_F16 || precision == PFDTD_BF16 || precision == PFDTD_F16_DIFF)
+    c->scale_req = scale_exp;
   return zero_fields(c);
 }
 
@@ -1090,9 +1103,14 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   if (n_steps < 0) return fail(PFDTD_E_ARG, "negative step count");
   rc = check_runnable(c);
   if (rc) return rc;
-  const int scale = (precision == PFDTD_F16 || precision == PFDTD_BF16) ? c->scale_req : 0;
+  const int scale = (precision == PFDTD_F16 || precision == PFDTD_BF16 ||
+                     precision == PFDTD_F16_DIFF) ? c->scale_req : 0;
   rc = ensure_fields(c, precision, scale);
   if (rc) return rc;
+  const bool diff = precision == PFDTD_F16_DIFF;
+  if (diff && (!fused_eligible(c) || c->scheme != 0 || c->X % 2 != 0))
+    return fail(PFDTD_E_STATE, "the difference form runs through the two-column sweep only: forward "
+                "update, X even, no batch, no source on the slices z = 0 or Z-1");
   rc = zero_fields(c);
   if (rc) return rc;
   rc = ensure_responses(c, n_steps);
@@ -1109,13 +1127,19 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
 
   HIP_TRY(hipEventRecord(c->ev_begin, c->stream));
   // step 0's source samples go into the (zero) current field before the first update
-  if (n_steps > 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, 0), c->stream));
+  if (n_steps > 0) {
+    InjectArgs ia = inject_args(c, -1, 0);
+    if (diff) ia.dfield = c->d_Pp;
+    HIP_TRY(launch_inject_gather(p_storage(c->precision), c->d_P, ia, c->stream));
+  }
 
   bool fused = fused_eligible(c);
   rc = ensure_pm(c, &fused);
   if (rc) return rc;
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
-                                  c->fshape, c->scheme);
+                                  diff ? PFDTD_SWEEP_PAIR : c->fshape, c->scheme);
+  if (diff && (!fused || !fg.pair))
+    return fail(PFDTD_E_NOMEM, "no room for the packed plane the two-column sweep reads");
   int done = 0;
   auto wall0 = std::chrono::steady_clock::now();
   while (done < n_steps) {
@@ -1131,11 +1155,24 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
         ++nev;
         // P^{step+2} (d_P2): gather step+1, inject step+2
         const int next = step + 2 < n_steps ? step + 2 : -1;
-        HIP_TRY(launch_inject_gather(c->precision, c->d_P2, inject_args(c, step + 1, next), c->stream));
+        InjectArgs ia = inject_args(c, step + 1, next);
+        if (diff) ia.dfield = c->d_Pp2;
+        HIP_TRY(launch_inject_gather(p_storage(c->precision), c->d_P2, ia, c->stream));
         std::swap(c->d_P, c->d_P2);
         std::swap(c->d_Pp, c->d_Pp2);
         k += 2;
         c->upd_launches += 2;
+      } else if (diff) {
+        // the difference form's odd last step: one level of the sweep (gather of `step` and the
+        // injection of step+1 happen inside it)
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev], c->stream));
+        HIP_TRY(launch_update2_any(c, fg, step, 1));
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev + 1], c->stream));
+        ++nev;
+        std::swap(c->d_P, c->d_P2);
+        std::swap(c->d_Pp, c->d_Pp2);
+        k += 1;
+        c->upd_launches += 1;
       } else {
         if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev], c->stream));
         HIP_TRY(launch_update_any(c, g));
@@ -1180,6 +1217,8 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
 }
 
 int pfdtd_begin_steps(pfdtd_ctx *c, int precision, int n_steps) {
+  if (precision == PFDTD_F16_DIFF)
+    return fail(PFDTD_E_ARG, "the difference form (PFDTD_F16_DIFF) runs through pfdtd_run only");
   int rc = check_ctx(c);
   if (rc) return rc;
   if (n_steps < 0) return fail(PFDTD_E_ARG, "negative step count");
@@ -1327,7 +1366,7 @@ int pfdtd_capture_slice(pfdtd_ctx *c, int64_t slice, int orientation, float *p_o
   HIP_TRY(hipMalloc((void **)&d_p, sizeof(float) * n));
   hipError_t e = hipMalloc((void **)&d_q, (size_t)n);
   if (e == hipSuccess)
-    e = launch_capture_slice(c->precision, c->d_P, c->d_pos, (int)c->X, (int)c->Y, (int)c->Z,
+    e = launch_capture_slice(p_storage(c->precision), c->d_P, c->d_pos, (int)c->X, (int)c->Y, (int)c->Z,
                              slice, orientation, c->scale_exp, d_p, d_q, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(p_out, d_p, sizeof(float) * n, hipMemcpyDeviceToHost);
@@ -1350,7 +1389,7 @@ int pfdtd_capture_slice_noalloc(pfdtd_ctx *c, int64_t slice, int orientation, fl
   if (!base) return fail(PFDTD_E_STATE, "static capture buffer not resolved on device %d", c->device);
   float *d_p = (float *)base;
   uint8_t *d_q = base + (size_t)PFDTD_STATIC_CAPTURE_MAX * sizeof(float);
-  hipError_t e = launch_capture_slice(c->precision, c->d_P, c->d_pos, (int)c->X, (int)c->Y,
+  hipError_t e = launch_capture_slice(p_storage(c->precision), c->d_P, c->d_pos, (int)c->X, (int)c->Y,
                                       (int)c->Z, slice, orientation, c->scale_exp, d_p, d_q,
                                       c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(p_out, d_p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
@@ -1373,7 +1412,7 @@ int pfdtd_capture_field(pfdtd_ctx *c, float *out, int64_t capacity) {
   }
   float *d_tmp = nullptr;
   HIP_TRY(hipMalloc((void **)&d_tmp, sizeof(float) * c->N));
-  hipError_t e = launch_field_to_f32(c->precision, c->d_P, c->N, c->scale_exp, d_tmp, c->stream);
+  hipError_t e = launch_field_to_f32(p_storage(c->precision), c->d_P, c->N, c->scale_exp, d_tmp, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(out, d_tmp, sizeof(float) * c->N, hipMemcpyDeviceToHost);
   (void)hipFree(d_tmp);

@@ -313,10 +313,20 @@ __device__ __forceinline__ void inject_gather_block(S *field, const InjectArgs &
         smp = (C)a.sig_f[off];
     }
     smp = smp * scale;
-    if (a.src_kind[s] == 0 || a.soft_overwrite)
+    const bool hard = a.src_kind[s] == 0 || a.soft_overwrite;
+    if (a.dfield) {
+      // the difference form: the sample changes P and keeps the past value P - D
+      S *dfield = (S *)a.dfield;
+      const C p = Fmt<S>::ld(field, idx);
+      const C past = p - Fmt<S>::ld(dfield, idx);
+      const C pn = Fmt<S>::round(hard ? smp : p + smp);
+      Fmt<S>::st(field, idx, pn);
+      Fmt<S>::st(dfield, idx, pn - past);
+    } else if (hard) {
       Fmt<S>::st(field, idx, smp);
-    else
+    } else {
       Fmt<S>::st(field, idx, Fmt<S>::ld(field, idx) + smp);
+    }
   };
   if (a.distinct) {
     // distinct elements: the order of application cannot change any value

@@ -625,8 +625,14 @@ __device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, u
 constexpr int kPairTY = 4;
 constexpr int kPairWY = 4;
 
-template <typename S>
-__global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
+// DIFF: the difference form (PFDTD_F16_DIFF). A holds P^n and B holds D^n = P^n - P^{n-1}; the
+// past value of a node is P^n - D^n (FP32), and each level stores fl16(ret) and fl16(ret - P),
+// ret the FP32 result: P^{n+2} into Dn and D^{n+2} into Cn (both at the second level), or with
+// one level (a.levels == 1) P^{n+1} into Dn and D^{n+1} into Cn. A source sample keeps the past
+// value: D = fl16(P_new - (P_old - D_old)). The level-1 differences of the owned rows stay in
+// registers for the second level (dcur, dprev), so this variant runs at three waves per SIMD.
+template <typename S, bool DIFF>
+__global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_kernel(
     Fused2Ctx<S, CoefF, float> p) {
 #pragma clang fp contract(off)
   typedef float C;
@@ -652,6 +658,7 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
   const int yt = (byi * WY + wave) * TY;  // first owned row
   const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);
+  const bool two = !DIFF || a.levels >= 2;  // second time level (the difference form may take one)
 
   extern __shared__ __align__(16) unsigned char s_dyn[];
   CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
@@ -728,6 +735,9 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
   };
   constexpr uint64_t kAllRows = (1ull << NQ) - 1ull;
   uint64_t ar_win = air_window(z0 - 1);
+  Pair2 dprev[TY];  // DIFF: D^{n+1} of the owned rows at the slice the second level updates
+#pragma unroll
+  for (int o = 0; o < TY; ++o) dprev[o] = Pair2{0.f, 0.f};
 
   // P^n rows as (column x0, column x0+1) pairs of FP32 values, P^{n+1} likewise, packed
   // position|material of both columns in one register per row.
@@ -784,6 +794,7 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
     for (int r = 0; r < NA; ++r) ap[r] = Pair2{lo_half<S>(ua[r]), hi_half<S>(ua[r])};
 
     // ---- first time level: P^{n+1}[k]
+    Pair2 dcur[TY];  // DIFF: D^{n+1}[k] of the owned rows
     auto level1 = [&](auto table) {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) {
@@ -798,11 +809,19 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
         s1 = s1 + ac[r - 1].y;
         s1 = s1 + lane_next(ac[r].x);        // x+1: the next lane's lower column
         s1 = s1 + ac[r].x;                   // x-1
-        const C v0 = Fmt<S>::round(node(table, bc[j] & 0xFFFFu, ac[r].x, s0, lo_half<S>(pu[j])));
-        const C v1 = Fmt<S>::round(node(table, bc[j] >> 16, ac[r].y, s1, hi_half<S>(pu[j])));
+        const C pp0 = DIFF ? ac[r].x - lo_half<S>(pu[j]) : lo_half<S>(pu[j]);
+        const C pp1 = DIFF ? ac[r].y - hi_half<S>(pu[j]) : hi_half<S>(pu[j]);
+        const C w0 = node(table, bc[j] & 0xFFFFu, ac[r].x, s0, pp0);
+        const C w1 = node(table, bc[j] >> 16, ac[r].y, s1, pp1);
         const bool row = upd1 && ((ru >> r) & 1u);
-        q2[j].x = (row && cu0) ? v0 : (C)0;
-        q2[j].y = (row && cu1) ? v1 : (C)0;
+        q2[j].x = (row && cu0) ? Fmt<S>::round(w0) : (C)0;
+        q2[j].y = (row && cu1) ? Fmt<S>::round(w1) : (C)0;
+        if constexpr (DIFF) {
+          if (j >= 1 && j <= TY) {
+            dcur[j - 1].x = (row && cu0) ? Fmt<S>::round(w0 - ac[r].x) : (C)0;
+            dcur[j - 1].y = (row && cu1) ? Fmt<S>::round(w1 - ac[r].y) : (C)0;
+          }
+        }
       }
     };
     if (!air_fast || (!air1 && any_nonair(bc))) {
@@ -842,25 +861,43 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
 #pragma unroll
         for (int j = 0; j < NQ; ++j)
           if (q[1] == yt - 1 + j) {
-            const C v = hard ? Fmt<S>::round(smp) : Fmt<S>::round(pick(q2[j], hi) + smp);
+            const C old = pick(q2[j], hi);
+            const C v = hard ? Fmt<S>::round(smp) : Fmt<S>::round(old + smp);
             q2[j].x = hi ? q2[j].x : v;
             q2[j].y = hi ? v : q2[j].y;
+            if constexpr (DIFF) {
+              if (j >= 1 && j <= TY) {  // owned rows: keep the past value
+                const C past = old - pick(dcur[j - 1], hi);
+                const C d = Fmt<S>::round(v - past);
+                dcur[j - 1].x = hi ? dcur[j - 1].x : d;
+                dcur[j - 1].y = hi ? d : dcur[j - 1].y;
+              }
+            }
           }
       }
     }
 
-    // ---- store the owned part of P^{n+1}[k]
-    if (k >= z0 && k < z1) {
-      const auto rC = slice_rsrc(gC + (size_t)k * XY, sbytes);
+    // ---- store the owned part of P^{n+1}[k] (the difference form with one level: P^{n+1}
+    // into Dn and D^{n+1} into Cn; with two levels it stores at the second level only)
+    if (k >= z0 && k < z1 && (!DIFF || !two)) {
+      const auto rC = slice_rsrc((DIFF ? gD : gC) + (size_t)k * XY, sbytes);
 #pragma unroll
       for (int o = 0; o < TY; ++o)
         if (((rs >> o) & 1u) && owned_lane)
           bst_pair(rC, ro[o + 2], pack_halves<S>(q2[o + 1].x, q2[o + 1].y));
+      if constexpr (DIFF) {
+        const auto rDd = slice_rsrc(gC + (size_t)k * XY, sbytes);
+#pragma unroll
+        for (int o = 0; o < TY; ++o)
+          if (((rs >> o) & 1u) && owned_lane)
+            bst_pair(rDd, ro[o + 2], pack_halves<S>(dcur[o].x, dcur[o].y));
+      }
     }
 
     // ---- second time level: P^{n+2}[k-1] on the owned rows
-    if (k - 1 >= z0) {
+    if (two && k - 1 >= z0) {
       const auto rD = slice_rsrc(gD + (size_t)(k - 1) * XY, sbytes);
+      const auto rCd = slice_rsrc(gC + (size_t)(k - 1) * XY, sbytes);
       auto level2 = [&](auto table) {
 #pragma unroll
         for (int o = 0; o < TY; ++o) {
@@ -875,9 +912,16 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
           s1 = s1 + q1[j - 1].y;
           s1 = s1 + lane_next(q1[j].x);
           s1 = s1 + q1[j].x;
-          C v0 = node(table, bp[j] & 0xFFFFu, q1[j].x, s0, am[o + 2].x);
-          C v1 = node(table, bp[j] >> 16, q1[j].y, s1, am[o + 2].y);
+          const C pp0 = DIFF ? q1[j].x - dprev[o].x : am[o + 2].x;
+          const C pp1 = DIFF ? q1[j].y - dprev[o].y : am[o + 2].y;
+          C v0 = node(table, bp[j] & 0xFFFFu, q1[j].x, s0, pp0);
+          C v1 = node(table, bp[j] >> 16, q1[j].y, s1, pp1);
           const bool row = (ru >> (o + 2)) & 1u;
+          if constexpr (DIFF) {
+            const C e0 = (row && cu0) ? Fmt<S>::round(v0 - q1[j].x) : (C)0;
+            const C e1 = (row && cu1) ? Fmt<S>::round(v1 - q1[j].y) : (C)0;
+            if (((rs >> o) & 1u) && owned_lane) bst_pair(rCd, ro[o + 2], pack_halves<S>(e0, e1));
+          }
           v0 = (row && cu0) ? v0 : (C)0;
           v1 = (row && cu1) ? v1 : (C)0;
           if (((rs >> o) & 1u) && owned_lane) bst_pair(rD, ro[o + 2], pack_halves<S>(v0, v1));
@@ -888,6 +932,10 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
       } else {
         level2(std::false_type());
       }
+    }
+    if constexpr (DIFF) {
+#pragma unroll
+      for (int o = 0; o < TY; ++o) dprev[o] = dcur[o];
     }
   };
 
@@ -1031,7 +1079,13 @@ static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a
   const Fused2Ctx<S, CT, C> p{a, g.X, g.Y, g.Z, g.zc};
   if constexpr (sizeof(S) == 2) {
     if (g.pair) {
-      hipLaunchKernelGGL((srl_update2_pair_kernel<S>), g.grid, g.block, lds, stream, p);
+      if constexpr (std::is_same<S, h16>::value) {
+        if (a.diff) {
+          hipLaunchKernelGGL((srl_update2_pair_kernel<S, true>), g.grid, g.block, lds, stream, p);
+          return hipGetLastError();
+        }
+      }
+      hipLaunchKernelGGL((srl_update2_pair_kernel<S, false>), g.grid, g.block, lds, stream, p);
       return hipGetLastError();
     }
   }

@@ -183,6 +183,8 @@ struct Fused2Args {
   double *resp;
   int resp_steps;
   int gather_step;
+  int levels = 2;  // time levels of this launch (1 only for the difference form's last step)
+  int diff = 0;    // 1: the difference form (B holds D = P^n - P^{n-1}; 16-bit storage only)
 };
 struct FusedGeom {
   int X, Y, Z;
@@ -230,6 +232,7 @@ struct InjectArgs {
   int soft_overwrite;
   int scale_exp;
   int distinct;  // 1: no two sources share an element, so they may be applied in parallel
+  void *dfield = nullptr;  // the difference form's D field (injection keeps P - D), or null
 };
 hipError_t launch_inject_gather(int precision, void *field, const InjectArgs &a,
                                 hipStream_t stream);

@@ -16,7 +16,7 @@ LIB_PATH = Path(os.environ.get("PFDTD_LIB", Path(__file__).resolve().parent / "l
 OK = 0
 E_ARG, E_HIP, E_NOMEM, E_STATE, E_DOMAIN = -1, -2, -3, -4, -5
 
-F32, F64, F16, BF16 = 0, 1, 2, 3
+F32, F64, F16, BF16, F16_DIFF = 0, 1, 2, 3, 4
 SWEEP_AUTO, SWEEP_COLUMN, SWEEP_WIDE, SWEEP_PAIR = 0, 1, 2, 3
 SRC_HARD, SRC_SOFT, SRC_TRANSPARENT = 0, 1, 2
 SRL_FORWARD, SHARED, SRL = 0, 1, 2  # enum UpdateType

@@ -3,7 +3,8 @@ every receiver response reported as relative L2 and peak error against the FP64 
 
 Storage formats: FP32 fields and arithmetic; FP16 and BF16 fields with FP32 arithmetic and a
 power-of-two scale 2^e chosen so the expected field peak maps to about 2^12 (the source peak
-sets it: a unit pulse gives e = 12). Errors per receiver r:
+sets it: a unit pulse gives e = 12); and `f16diff`, binary16 storage of P^n and
+D^n = P^n - P^{n-1} (the difference form, PFDTD_F16_DIFF; rooms of even X). Errors per receiver r:
   rel_l2[r] = |x_r - ref_r|_2 / |ref_r|_2,   peak[r] = max |x_r - ref_r| / max |ref_r|.
 The summary statistics cover the receivers the wave has reached within the run, those whose
 FP64 peak is at least REACHED (1e-4) times the largest receiver peak. A receiver outside the
@@ -36,7 +37,7 @@ from .solver import Solver
 POST_FS = 40000.0
 POST_FRAC = 0.2
 
-FORMATS = {"f32": N.F32, "f16": N.F16, "bf16": N.BF16}
+FORMATS = {"f32": N.F32, "f16": N.F16, "f16diff": N.F16_DIFF, "bf16": N.BF16}
 
 
 def scale_exponent(peak: float, target_log2: int = 12) -> int:
@@ -64,7 +65,7 @@ def errors(x: np.ndarray, ref: np.ndarray) -> dict:
             "rel_l2_max_all": float(rel.max())}
 
 
-def study(scene, formats=("f32", "f16", "bf16"), device: int = 0, log=print) -> dict:
+def study(scene, formats=("f32", "f16", "f16diff", "bf16"), device: int = 0, log=print) -> dict:
     """Run `scene` in FP64 and in each storage format; return the error report."""
     out = {"scene": scene.name, "nodes": scene.n_nodes, "steps": scene.n_steps,
            "receivers": int(len(scene.rec_xyz)), "formats": {}}
@@ -79,7 +80,7 @@ def study(scene, formats=("f32", "f16", "bf16"), device: int = 0, log=print) -> 
         for f in formats:
             prec = FORMATS[f]
             e = 0
-            if prec in (N.F16, N.BF16):
+            if prec in (N.F16, N.BF16, N.F16_DIFF):
                 e = scale_exponent(src_peak)
                 s.prepare_fields(prec, e)
             t0 = time.perf_counter()

@@ -105,7 +105,7 @@ def test_binding_constants_match_header():
     """The Python binding's numeric constants are the header's."""
     d = header_defines()
     pairs = {"PFDTD_F32": native.F32, "PFDTD_F64": native.F64, "PFDTD_F16": native.F16,
-             "PFDTD_BF16": native.BF16, "PFDTD_SWEEP_AUTO": native.SWEEP_AUTO,
+             "PFDTD_BF16": native.BF16, "PFDTD_F16_DIFF": native.F16_DIFF, "PFDTD_SWEEP_AUTO": native.SWEEP_AUTO,
              "PFDTD_SWEEP_COLUMN": native.SWEEP_COLUMN, "PFDTD_SWEEP_WIDE": native.SWEEP_WIDE,
              "PFDTD_SWEEP_PAIR": native.SWEEP_PAIR,
              "PFDTD_OK": native.OK, "PFDTD_E_ARG": native.E_ARG, "PFDTD_E_HIP": native.E_HIP,
