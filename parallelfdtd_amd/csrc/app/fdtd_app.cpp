@@ -304,8 +304,9 @@ void App::check(int rc, const char *what) {
   }
 }
 
-// initializeDevices, src/App.cpp:125-131: pick the device with the most free memory (the
-// reference picks the highest-GFLOPS one) and make it current; devices are not reset.
+// initializeDevices, src/App.cpp:125-131: one context on one device, the first visible one
+// (HIP_VISIBLE_DEVICES selects it; the reference queries and resets every device and picks the
+// highest-GFLOPS one). Only that device is touched, and it is not reset.
 void App::initializeDevices() {
   int n = 0;
   check(pfdtd_device_count(&n), "initializeDevices");
@@ -313,14 +314,7 @@ void App::initializeDevices() {
     log_line("initializeDevices", "no device");
     throw(-1);
   }
-  uint64_t best = 0;
-  for (int d = 0; d < n; ++d) {
-    uint64_t f = 0, t = 0;
-    if (pfdtd_device_mem(d, &f, &t) == PFDTD_OK && f > best) {
-      best = f;
-      device_ = d;
-    }
-  }
+  device_ = 0;
   if (ctx_) {
     pfdtd_ctx_destroy(ctx_);
     ctx_ = nullptr;
