@@ -181,8 +181,19 @@ int pfdtd_set_update_type(pfdtd_ctx *ctx, int type);
  * them after pfdtd_set_sources / pfdtd_set_receivers). Each slot evolves exactly as an
  * unbatched run with only its own sources would. No reference counterpart: the reference runs
  * one source configuration per simulation (App::runSimulation, src/App.cpp:308-348). Batched
- * runs use the single-level update; slice/field capture is unavailable (raw fields are). */
+ * runs in the interleaved layout use the single-level update; slice/field capture is
+ * unavailable (raw fields are). */
 int pfdtd_set_batch(pfdtd_ctx *ctx, int slots);
+/* Field layout of a batch: PFDTD_BATCH_INTERLEAVED (default; slot innermost, element
+ * voxel * slots + slot, advanced by the single-level batched update that reads each voxel's
+ * position and material bytes once for all slots) or PFDTD_BATCH_GROUPED (slot outermost,
+ * element slot * N + voxel: each slot's fields have the unbatched layout, and one launch of the
+ * two-level sweep advances every slot, the slots of one tile in consecutive blocks so that the
+ * tile's packed byte plane and air-row bits are read once into one XCD's L2). Results per slot
+ * are the same in both. No reference counterpart. */
+#define PFDTD_BATCH_INTERLEAVED 0
+#define PFDTD_BATCH_GROUPED 1
+int pfdtd_set_batch_layout(pfdtd_ctx *ctx, int layout);
 int pfdtd_set_source_slots(pfdtd_ctx *ctx, int n_src, const int32_t *slot);
 int pfdtd_set_receiver_slots(pfdtd_ctx *ctx, int n_rec, const int32_t *slot);
 

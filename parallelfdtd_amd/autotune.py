@@ -1,7 +1,9 @@
 """Autotuner for batched runs: sweeps the batch width B (sources advanced per launch, one slot
-each, fields interleaved source-innermost), the update tile (y rows per thread; a 256-thread
-block covers 256/B x-columns of one row band) and the z-chunk length, timing a short run of
-each candidate on the real domain.
+each) in both field layouts -- interleaved source-innermost, advanced by the single-level
+batched update (its tile: y rows per thread, a 256-thread block covering 256/B x-columns of one
+row band, and the z-chunk length), and grouped source-outermost, advanced by the two-level
+sweep in one launch for all slots (block shape and z-chunk) -- timing a short run of each
+candidate on the real domain.
 
 Throughput is per source: voxel-updates/s summed over the B slots,
   per_source_gvox = N_nodes * B * steps / seconds / 1e9,
@@ -59,11 +61,12 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
     out = {"scene": scene.name, "nodes": nodes, "steps": steps, "candidates": [], "best": {},
            "best_single_level": {}}
 
-    def record(B, levels, ry, zc, ms, shape=None):
+    def record(B, levels, ry, zc, ms, shape=None, layout="interleaved"):
         per_src = nodes * B / (ms * 1e-3) / 1e9
         gbs = nodes * (BYTES_F32["per_voxel"] + BYTES_F32["per_slot"] * B) / (ms * 1e-3) / 1e9
         # six significant digits: the speedups below are ratios of these values
-        cand = {"B": B, "levels": levels, "rows": ry, "zchunk": zc, "shape": shape,
+        cand = {"B": B, "layout": layout, "levels": levels, "rows": ry, "zchunk": zc,
+                "shape": shape,
                 "ms_per_step": float(f"{ms:.6g}"), "per_source_gvox": float(f"{per_src:.6g}"),
                 "alg_GBps": float(f"{gbs:.6g}")}
         out["candidates"].append(cand)
@@ -84,6 +87,19 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
                     if best is None or cand["per_source_gvox"] > best["per_source_gvox"]:
                         best = cand
             out["best_single_level"][str(B)] = dict(best)
+            if B > 1:
+                s.set_batch_layout(N.BATCH_GROUPED)
+                for name, shape in SHAPES.items():
+                    s.set_sweep_shape(shape)
+                    for zc in TWO_LEVEL_ZCHUNKS:
+                        s.set_time_levels(2, zc)
+                        cand = record(B, 2, None, zc, time_config(s, steps, warmup, N.F32), name,
+                                      "grouped")
+                        if cand["per_source_gvox"] > best["per_source_gvox"]:
+                            best = cand
+                s.set_sweep_shape(N.SWEEP_AUTO)
+                s.set_time_levels(1)
+                s.set_batch_layout(N.BATCH_INTERLEAVED)
             if B == 1:
                 for name, shape in SHAPES.items():
                     s.set_sweep_shape(shape)
@@ -101,7 +117,7 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
             for v in table.values():
                 v["per_source_speedup_vs_B1"] = float(f"{v['per_source_gvox'] / base['per_source_gvox']:.4g}")
     top = max(out["best"].values(), key=lambda c: c["per_source_gvox"])
-    out["choice"] = {k: top[k] for k in ("B", "levels", "rows", "zchunk", "shape")}
+    out["choice"] = {k: top[k] for k in ("B", "layout", "levels", "rows", "zchunk", "shape")}
     return out
 
 

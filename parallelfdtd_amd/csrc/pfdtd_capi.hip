@@ -116,9 +116,12 @@ struct pfdtd_ctx {
   double *d_sig_d = nullptr;
   int n_rec = 0;
   std::vector<int32_t> rec_xyz;
-  // batch: fields hold 2^lgB values per voxel (slot innermost); per-source/receiver slot
+  // batch: fields hold 2^lgB values per voxel, slot innermost, or (grouped) 2^lgB whole fields
+  // one after another; per-source/receiver slot
   int lgB = 0;
+  int grouped = 0;
   std::vector<int32_t> src_slot, rec_slot;
+  int32_t *d_src_slot = nullptr, *d_rec_slot = nullptr;
   int64_t *d_rec_idx = nullptr;
   int32_t *d_rec_xyz = nullptr;
 
@@ -273,10 +276,14 @@ int64_t lin_idx(const pfdtd_ctx *c, const int32_t *q) {
   return ((int64_t)q[2] * c->Y + q[1]) * c->X + q[0];
 }
 
+// Slots one two-level launch advances (grouped batch), 1 otherwise.
+int batch_slots(const pfdtd_ctx *c) { return c->grouped ? 1 << c->lgB : 1; }
+
 // Element index of a source/receiver in the (possibly batched) field, -1 outside the grid.
 int64_t elem_idx(const pfdtd_ctx *c, const int32_t *q, int slot) {
   const int64_t v = lin_idx(c, q);
-  return v < 0 ? -1 : (v << c->lgB) + slot;
+  if (v < 0) return -1;
+  return c->grouped ? (int64_t)slot * c->N + v : (v << c->lgB) + slot;
 }
 
 // 1 when no two in-grid sources share a field element (injection order then cannot matter).
@@ -439,7 +446,7 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
     return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
   if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
   const size_t bytes = es * ((size_t)c->N << c->lgB);
-  const bool want4 = c->lgB == 0 && levels_for(c, precision) == 2;
+  const bool want4 = (c->lgB == 0 || c->grouped) && levels_for(c, precision) == 2;
   if (c->d_P && c->field_bytes == bytes && (!want4 || c->d_P2)) {
     c->precision = precision;
     c->scale_exp = scale_exp;
@@ -469,11 +476,17 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
 int zero_fields(pfdtd_ctx *c) {
   HIP_TRY(hipMemsetAsync(c->d_P, 0, c->field_bytes, c->stream));
   HIP_TRY(hipMemsetAsync(c->d_Pp, 0, c->field_bytes, c->stream));
-  const size_t slice = c->field_bytes / (size_t)c->Z;
+  // a grouped batch holds one whole field per slot, each with its own first and last slice
+  const int slots = batch_slots(c);
+  const size_t per_slot = c->field_bytes / (size_t)slots;
+  const size_t slice = per_slot / (size_t)c->Z;
   for (void *q : {c->d_P2, c->d_Pp2}) {
     if (!q) continue;
-    HIP_TRY(hipMemsetAsync(q, 0, slice, c->stream));
-    HIP_TRY(hipMemsetAsync((char *)q + (size_t)(c->Z - 1) * slice, 0, slice, c->stream));
+    for (int sl = 0; sl < slots; ++sl) {
+      char *base = (char *)q + (size_t)sl * per_slot;
+      HIP_TRY(hipMemsetAsync(base, 0, slice, c->stream));
+      HIP_TRY(hipMemsetAsync(base + (size_t)(c->Z - 1) * slice, 0, slice, c->stream));
+    }
   }
   c->last_direction = 1;
   return PFDTD_OK;
@@ -523,7 +536,28 @@ int coef_cols(const pfdtd_ctx *c) {
   return 1;
 }
 
+hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g);
+
+// Grouped batch: one single-level update of every slot (each slot's fields are one domain).
+hipError_t launch_update_grouped(pfdtd_ctx *c, const UpdateGeom &g) {
+  const size_t slot_bytes = c->field_bytes >> c->lgB;
+  void *P = c->d_P, *Pp = c->d_Pp;
+  const int lg = c->lgB;
+  hipError_t e = hipSuccess;
+  c->lgB = 0;
+  for (int sl = 0; sl < (1 << lg) && e == hipSuccess; ++sl) {
+    c->d_P = (char *)P + (size_t)sl * slot_bytes;
+    c->d_Pp = (char *)Pp + (size_t)sl * slot_bytes;
+    e = launch_update_any(c, g);
+  }
+  c->lgB = lg;
+  c->d_P = P;
+  c->d_Pp = Pp;
+  return e;
+}
+
 hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
+  if (c->grouped && c->lgB > 0) return launch_update_grouped(c, g);
   const int ncol = coef_cols(c);
   switch (c->precision) {
   case PFDTD_F32:
@@ -550,7 +584,7 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
 // The two-level update needs the spare pair and no source on the never-updated slices
 // z = 0 and z = Z-1 (their values would have to be carried between the four buffers).
 bool fused_eligible(const pfdtd_ctx *c) {
-  if (c->lgB > 0 || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
+  if ((c->lgB > 0 && !c->grouped) || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
     return false;
   // the two-level kernel addresses each z-slice through a buffer descriptor (32-bit range)
   const uint64_t esz = c->precision == PFDTD_F64 ? 8 : (c->precision == PFDTD_F32 ? 4 : 2);
@@ -586,6 +620,12 @@ Fused2Args<S, CT, C> fused_args(const pfdtd_ctx *c, const CT *coef, CT air, C la
   a.inject_step = (c->n_src > 0 && step + 1 < c->resp_steps) ? step + 1 : -1;
   a.levels = 2;
   a.diff = c->precision == PFDTD_F16_DIFF ? 1 : 0;
+  if (c->grouped && c->lgB > 0) {
+    a.n_slots = 1 << c->lgB;
+    a.slot_elems = c->N;
+    a.src_slot = c->d_src_slot;
+    a.rec_slot = c->d_rec_slot;
+  }
   a.soft_overwrite = (c->flags & PFDTD_SOFT_OVERWRITE) ? 1 : 0;
   a.scale_exp = c->scale_exp;
   a.rec_xyz = c->d_rec_xyz;
@@ -696,6 +736,8 @@ int pfdtd_ctx_destroy(pfdtd_ctx *c) {
   dfree(c->d_src_idx);
   dfree(c->d_src_xyz);
   dfree(c->d_src_kind);
+  dfree(c->d_src_slot);
+  dfree(c->d_rec_slot);
   dfree(c->d_sig_f);
   dfree(c->d_sig_d);
   dfree(c->d_rec_idx);
@@ -981,6 +1023,7 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
   dfree(c->d_src_idx);
   dfree(c->d_src_xyz);
   dfree(c->d_src_kind);
+  dfree(c->d_src_slot);
   dfree(c->d_sig_f);
   dfree(c->d_sig_d);
   c->n_src = 0;
@@ -1001,6 +1044,8 @@ int pfdtd_set_sources(pfdtd_ctx *c, int n_src, const int32_t *xyz, const int32_t
     for (int s = 0; s < n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], 0);
   HIP_TRY(hipMalloc((void **)&c->d_src_idx, sizeof(int64_t) * n_src));
   HIP_TRY(hipMalloc((void **)&c->d_src_kind, sizeof(int32_t) * n_src));
+  HIP_TRY(hipMalloc((void **)&c->d_src_slot, sizeof(int32_t) * n_src));
+  HIP_TRY(hipMemcpy(c->d_src_slot, c->src_slot.data(), sizeof(int32_t) * n_src, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc((void **)&c->d_src_xyz, sizeof(int32_t) * 3 * n_src));
   {
     int rc2 = upload_xyz(c, c->src_xyz, c->d_src_xyz);
@@ -1028,6 +1073,7 @@ int pfdtd_set_receivers(pfdtd_ctx *c, int n_rec, const int32_t *xyz) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   dfree(c->d_rec_idx);
   dfree(c->d_rec_xyz);
+  dfree(c->d_rec_slot);
   c->n_rec = 0;
   c->rec_xyz.assign(xyz, xyz + 3 * (size_t)n_rec);
   c->rec_slot.assign((size_t)n_rec, 0);
@@ -1038,6 +1084,8 @@ int pfdtd_set_receivers(pfdtd_ctx *c, int n_rec, const int32_t *xyz) {
     for (int r = 0; r < n_rec; ++r) idx[r] = elem_idx(c, &c->rec_xyz[3 * r], 0);
   HIP_TRY(hipMalloc((void **)&c->d_rec_idx, sizeof(int64_t) * n_rec));
   HIP_TRY(hipMalloc((void **)&c->d_rec_xyz, sizeof(int32_t) * 3 * n_rec));
+  HIP_TRY(hipMalloc((void **)&c->d_rec_slot, sizeof(int32_t) * n_rec));
+  HIP_TRY(hipMemcpy(c->d_rec_slot, c->rec_slot.data(), sizeof(int32_t) * n_rec, hipMemcpyHostToDevice));
   {
     int rc2 = upload_xyz(c, c->rec_xyz, c->d_rec_xyz);
     if (rc2) return rc2;
@@ -1108,7 +1156,7 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   rc = ensure_fields(c, precision, scale);
   if (rc) return rc;
   const bool diff = precision == PFDTD_F16_DIFF;
-  if (diff && (!fused_eligible(c) || c->scheme != 0 || c->X % 2 != 0))
+  if (diff && (!fused_eligible(c) || c->scheme != 0 || c->X % 2 != 0 || c->lgB > 0))
     return fail(PFDTD_E_STATE, "the difference form runs through the two-column sweep only: forward "
                 "update, X even, no batch, no source on the slices z = 0 or Z-1");
   rc = zero_fields(c);
@@ -1118,7 +1166,8 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   if (steps_done) *steps_done = 0;
   c->upd_ms = 0.0;
   c->upd_launches = 0;
-  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc, c->lgB);
+  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc,
+                                          c->grouped ? 0 : c->lgB);
   const int bundle = c->poll_interval > 0 ? c->poll_interval : kProgressMod;
   if (c->kernel_timing) {
     rc = ensure_events(c, 2 * (size_t)bundle);
@@ -1137,7 +1186,7 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   rc = ensure_pm(c, &fused);
   if (rc) return rc;
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
-                                  diff ? PFDTD_SWEEP_PAIR : c->fshape, c->scheme);
+                                  diff ? PFDTD_SWEEP_PAIR : c->fshape, c->scheme, batch_slots(c));
   if (diff && (!fused || !fg.pair))
     return fail(PFDTD_E_NOMEM, "no room for the packed plane the two-column sweep reads");
   int done = 0;
@@ -1242,7 +1291,8 @@ int pfdtd_step(pfdtd_ctx *c, int step, int direction) {
   // launchFDTD3dStep: inject this step's samples, update, flip unless the direction changed,
   // then read receivers into response[step] (kernels3d.cu:390-475).
   if (step >= 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, step), c->stream));
-  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc, c->lgB);
+  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc,
+                                          c->grouped ? 0 : c->lgB);
   HIP_TRY(launch_update_any(c, g));
   if (c->last_direction == direction) std::swap(c->d_P, c->d_Pp);
   c->last_direction = direction;
@@ -1271,12 +1321,13 @@ int pfdtd_steps(pfdtd_ctx *c, int first_step, int n_steps) {
     --n_steps;
     if (n_steps == 0) return PFDTD_OK;
   }
-  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc, c->lgB);
+  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc,
+                                          c->grouped ? 0 : c->lgB);
   bool fused = fused_eligible(c);
   rc = ensure_pm(c, &fused);
   if (rc) return rc;
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
-                                  c->fshape, c->scheme);
+                                  c->fshape, c->scheme, batch_slots(c));
   // the sequence of pfdtd_step calls: inject(s), update, gather(s) for s = first .. first+n-1,
   // with the injection of each following step applied right after the previous gather
   HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, first_step), c->stream));
@@ -1483,13 +1534,16 @@ static int refresh_indices(pfdtd_ctx *c) {
     std::vector<int64_t> idx(c->n_src);
     for (int s = 0; s < c->n_src; ++s) idx[s] = elem_idx(c, &c->src_xyz[3 * s], c->src_slot[s]);
     c->src_distinct = distinct_indices(idx);
-    c->src_distinct = distinct_indices(idx);
     HIP_TRY(hipMemcpy(c->d_src_idx, idx.data(), sizeof(int64_t) * c->n_src, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_src_slot, c->src_slot.data(), sizeof(int32_t) * c->n_src,
+                      hipMemcpyHostToDevice));
   }
   if (c->n_rec) {
     std::vector<int64_t> idx(c->n_rec);
     for (int r = 0; r < c->n_rec; ++r) idx[r] = elem_idx(c, &c->rec_xyz[3 * r], c->rec_slot[r]);
     HIP_TRY(hipMemcpy(c->d_rec_idx, idx.data(), sizeof(int64_t) * c->n_rec, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_rec_slot, c->rec_slot.data(), sizeof(int32_t) * c->n_rec,
+                      hipMemcpyHostToDevice));
   }
   return PFDTD_OK;
 }
@@ -1506,6 +1560,17 @@ int pfdtd_set_batch(pfdtd_ctx *c, int slots) {
   c->lgB = lg;
   for (int32_t &s : c->src_slot) s = 0;
   for (int32_t &r : c->rec_slot) r = 0;
+  return refresh_indices(c);
+}
+
+int pfdtd_set_batch_layout(pfdtd_ctx *c, int layout) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (layout != PFDTD_BATCH_INTERLEAVED && layout != PFDTD_BATCH_GROUPED)
+    return fail(PFDTD_E_ARG, "batch layout %d not INTERLEAVED (0) or GROUPED (1)", layout);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (layout != c->grouped) free_fields(c);
+  c->grouped = layout;
   return refresh_indices(c);
 }
 

@@ -48,14 +48,17 @@ __device__ __forceinline__ bool in_box(const int32_t *q, int x0, int x1, int y0,
   return q[0] >= x0 && q[0] < x1 && q[1] >= y0 && q[1] < y1 && q[2] >= z0 && q[2] < z1;
 }
 
-// Order-preserving wave-level compaction of the entries of `xyz` inside a box into `list`.
-// Returns the count, or -1 when it exceeds the capacity (the caller then scans all entries).
+// Order-preserving wave-level compaction of the entries of `xyz` inside a box (and, with a slot
+// array, of batch slot `slot`) into `list`. Returns the count, or -1 when it exceeds the
+// capacity (the caller then scans all entries).
 __device__ __forceinline__ int wave_collect(const int32_t *xyz, int n, int x0, int x1, int y0,
-                                            int y1, int z0, int z1, int *list, int lane) {
+                                            int y1, int z0, int z1, int *list, int lane,
+                                            const int32_t *slotv = nullptr, int slot = 0) {
   int cnt = 0;
   for (int base = 0; base < n; base += 64) {
     const int s = base + lane;
-    const bool hit = s < n && in_box(xyz + 3 * s, x0, x1, y0, y1, z0, z1);
+    const bool hit = s < n && in_box(xyz + 3 * s, x0, x1, y0, y1, z0, z1) &&
+                     (!slotv || slotv[s] == slot);
     const uint64_t m = __ballot(hit);
     const int pre = __popcll(m & ((1ull << lane) - 1ull));
     if (hit && cnt >= 0 && cnt + pre < kFListCap) list[cnt + pre] = s;
@@ -210,6 +213,7 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
   Fused2Args<S, CT, C> a;
   int X, Y, Z, zc, bxi, byi, bzi;
   bool two = true;  // second time level (a pool job with one step left takes only the first)
+  int slot = 0;
   if constexpr (!POOL) {
     a = p.a;
     X = p.X;
@@ -219,10 +223,21 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
     const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
     const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
     const unsigned full = total & ~7u;
-    const unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+    unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+    // grouped batch: the slots of one spatial tile are consecutive, so one XCD's L2 serves
+    // the tile's packed plane and air-row bits to all of them
+    slot = (int)(tile % (unsigned)a.n_slots);
+    tile /= (unsigned)a.n_slots;
     bxi = (int)(tile % nx);
     byi = (int)((tile / nx) % ny);
     bzi = (int)(tile / (nx * ny));
+    if (slot) {
+      const int64_t off = (int64_t)slot * a.slot_elems;
+      a.A += off;
+      a.B += off;
+      a.Cn += off;
+      a.Dn += off;
+    }
   } else {
     // the flattened tiles of every job of the launch; the block finds its job by a binary
     // search over the launch table's cumulative tile counts
@@ -296,9 +311,9 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
   // Sources that land on this wave's P^{n+1} region (owned tile + 1-voxel ring, slices
   // z0-1 .. z1) and receivers on its owned tile.
   const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kFO + 1, yt - 1, yt + kFTY + 1,
-                                z0 - 1, z1 + 1, s_src[wave], lane);
+                                z0 - 1, z1 + 1, s_src[wave], lane, a.src_slot, slot);
   const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kFO, yt, yt + kFTY, z0, z1,
-                                s_rec[wave], lane);
+                                s_rec[wave], lane, a.rec_slot, slot);
   __syncthreads();
   // Waves whose tile lies past the grid stay in the z loop (every row masked) so that the
   // block's waves can step z together.
@@ -465,7 +480,9 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
         const int id = nrec < 0 ? t : s_rec[wave][t];
         const int32_t *q = a.rec_xyz + 3 * id;
         if (q[2] != k || q[0] != x) continue;
-        if (nrec < 0 && !in_box(q, xb, xb + kFO, yt, yt + kFTY, z0, z1)) continue;
+        if (nrec < 0 && (!in_box(q, xb, xb + kFO, yt, yt + kFTY, z0, z1) ||
+                         (a.rec_slot && a.rec_slot[id] != slot)))
+          continue;
 #pragma unroll
         for (int j = 0; j < NQ; ++j)
           if (q[1] == yt - 1 + j)
@@ -478,6 +495,7 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
         const int id = nsrc < 0 ? t : s_src[wave][t];
         const int32_t *q = a.src_xyz + 3 * id;
         if (q[2] != k || q[0] != x) continue;
+        if (nsrc < 0 && a.src_slot && a.src_slot[id] != slot) continue;
         C smp = (C)0;
         if (a.inject_step < a.n_samples) {
           const int64_t off = (int64_t)id * a.n_samples + a.inject_step;
@@ -643,13 +661,22 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
   __shared__ int s_rec[WY][kFListCap];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const Fused2Args<S, CoefF, float> &a = p.a;
+  Fused2Args<S, CoefF, float> a = p.a;
   const int X = p.X, Y = p.Y, Z = p.Z, zc = p.zc;
-  // XCD-aware tile order, as in srl_update2_kernel
+  // XCD-aware tile order, as in srl_update2_kernel (grouped-batch slots innermost)
   const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
   const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
   const unsigned full = total & ~7u;
-  const unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+  unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+  const int slot = (int)(tile % (unsigned)a.n_slots);
+  tile /= (unsigned)a.n_slots;
+  if (slot) {
+    const int64_t off = (int64_t)slot * a.slot_elems;
+    a.A += off;
+    a.B += off;
+    a.Cn += off;
+    a.Dn += off;
+  }
   const int xt = (int)(tile % nx);
   const int byi = (int)((tile / nx) % ny);
   const int bzi = (int)(tile / (nx * ny));
@@ -666,9 +693,9 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
   for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WY)
     s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
   const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kPO + 1, yt - 1, yt + TY + 1,
-                                z0 - 1, z1 + 1, s_src[wave], lane);
+                                z0 - 1, z1 + 1, s_src[wave], lane, a.src_slot, slot);
   const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kPO, yt, yt + TY, z0, z1,
-                                s_rec[wave], lane);
+                                s_rec[wave], lane, a.rec_slot, slot);
   __syncthreads();
 
   const size_t XY = (size_t)X * (size_t)Y;
@@ -837,7 +864,9 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
         const int id = nrec < 0 ? t : s_rec[wave][t];
         const int32_t *q = a.rec_xyz + 3 * id;
         if (q[2] != k || (q[0] & ~1) != x0) continue;
-        if (nrec < 0 && !in_box(q, xb, xb + kPO, yt, yt + TY, z0, z1)) continue;
+        if (nrec < 0 && (!in_box(q, xb, xb + kPO, yt, yt + TY, z0, z1) ||
+                         (a.rec_slot && a.rec_slot[id] != slot)))
+          continue;
         const bool hi = q[0] & 1;
 #pragma unroll
         for (int j = 0; j < NQ; ++j)
@@ -852,6 +881,7 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
         const int id = nsrc < 0 ? t : s_src[wave][t];
         const int32_t *q = a.src_xyz + 3 * id;
         if (q[2] != k || (q[0] & ~1) != x0) continue;
+        if (nsrc < 0 && a.src_slot && a.src_slot[id] != slot) continue;
         C smp = (C)0;
         if (a.inject_step < a.n_samples) smp = a.sig_f[(int64_t)id * a.n_samples + a.inject_step];
         smp = smp * scale;
@@ -1026,7 +1056,7 @@ hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int
 }
 
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int shape,
-                     int scheme) {
+                     int scheme, int slots) {
   const int kFTY = fused_ty_host(precision, scheme);
   FusedGeom g;
   g.X = X;
@@ -1045,19 +1075,20 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   if (g.pair) {
     const int rows = kPairTY * kPairWY;
     const int ptx = (X + kPO - 1) / kPO;
-    const int64_t pxy = (int64_t)ptx * ((Y + rows - 1) / rows);
+    const int64_t pxy = (int64_t)ptx * ((Y + rows - 1) / rows) * slots;
     if (zc < 1) {
       zc = 16;
       while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
     }
     g.zc = zc > zi ? zi : zc;
-    g.grid = dim3(ptx, (Y + rows - 1) / rows, (zi + g.zc - 1) / g.zc);
+    g.grid = dim3(ptx, (Y + rows - 1) / rows, ((zi + g.zc - 1) / g.zc) * slots);
     g.block = dim3(64 * kPairWY, 1, 1);
     return g;
   }
   const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
   const int WY = g.wide ? FusedShape<true>::WY : FusedShape<false>::WY;
-  const int64_t bxy = (int64_t)((ntx + WX - 1) / WX) * ((Y + kFTY * WY - 1) / (kFTY * WY));
+  const int64_t bxy =
+      (int64_t)((ntx + WX - 1) / WX) * ((Y + kFTY * WY - 1) / (kFTY * WY)) * slots;
   if (zc < 1) {
     // automatic: the longest of 32 (16 for 2-byte fields), 16 and 8 slices that still gives
     // 4096 blocks (16 per CU), so small rooms keep every CU busy to the end of the sweep; each
@@ -1067,7 +1098,8 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
     while (zc > 8 && bxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
   }
   g.zc = zc > zi ? zi : zc;
-  g.grid = dim3((ntx + WX - 1) / WX, (Y + kFTY * WY - 1) / (kFTY * WY), (zi + g.zc - 1) / g.zc);
+  g.grid = dim3((ntx + WX - 1) / WX, (Y + kFTY * WY - 1) / (kFTY * WY),
+                ((zi + g.zc - 1) / g.zc) * slots);
   g.block = dim3(64 * WX * WY, 1, 1);
   return g;
 }

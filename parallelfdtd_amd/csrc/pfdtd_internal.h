@@ -185,6 +185,12 @@ struct Fused2Args {
   int gather_step;
   int levels = 2;  // time levels of this launch (1 only for the difference form's last step)
   int diff = 0;    // 1: the difference form (B holds D = P^n - P^{n-1}; 16-bit storage only)
+  // Grouped batch (slot-outermost fields): the launch advances n_slots source configurations,
+  // slot s's fields at element offset s * slot_elems; sources and receivers carry their slot.
+  int n_slots = 1;
+  int64_t slot_elems = 0;
+  const int32_t *src_slot = nullptr;
+  const int32_t *rec_slot = nullptr;
 };
 struct FusedGeom {
   int X, Y, Z;
@@ -197,7 +203,7 @@ struct FusedGeom {
 // Grid of the two-level sweep; `pool`: serving-pool launches (1 x 4 blocks); `shape`: one of
 // PFDTD_SWEEP_AUTO / _COLUMN / _WIDE; `scheme`: 0 forward, 2 centred boundaries.
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false,
-                     int shape = 0, int scheme = 0);
+                     int shape = 0, int scheme = 0, int slots = 1);
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.

@@ -18,6 +18,7 @@ E_ARG, E_HIP, E_NOMEM, E_STATE, E_DOMAIN = -1, -2, -3, -4, -5
 
 F32, F64, F16, BF16, F16_DIFF = 0, 1, 2, 3, 4
 SWEEP_AUTO, SWEEP_COLUMN, SWEEP_WIDE, SWEEP_PAIR = 0, 1, 2, 3
+BATCH_INTERLEAVED, BATCH_GROUPED = 0, 1
 SRC_HARD, SRC_SOFT, SRC_TRANSPARENT = 0, 1, 2
 SRL_FORWARD, SHARED, SRL = 0, 1, 2  # enum UpdateType
 MATSLOT_INTENDED, MATSLOT_LITERAL = 0, 1
@@ -82,6 +83,7 @@ SIGNATURES: dict[str, list] = {
     "pfdtd_set_update_type": [_p, C.c_int],
     "pfdtd_transparent_signals": [_p, C.c_int, C.c_int, _p, C.c_int, _p, _p, _p, _p],
     "pfdtd_set_batch": [_p, C.c_int],
+    "pfdtd_set_batch_layout": [_p, C.c_int],
     "pfdtd_set_source_slots": [_p, C.c_int, _p],
     "pfdtd_set_receiver_slots": [_p, C.c_int, _p],
     "pfdtd_set_update_tiling": [_p, C.c_int, C.c_int],

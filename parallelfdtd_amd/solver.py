@@ -178,6 +178,11 @@ class Solver:
         self._c("pfdtd_set_batch", slots)
         self.slots = slots
 
+    def set_batch_layout(self, layout: int) -> None:
+        """N.BATCH_INTERLEAVED (slot innermost) or N.BATCH_GROUPED (slot outermost)."""
+        self._c("pfdtd_set_batch_layout", layout)
+        self.batch_layout = layout
+
     def set_source_slots(self, slots) -> None:
         sl = np.ascontiguousarray(slots, np.int32)
         self._c("pfdtd_set_source_slots", sl.size, N.ptr(sl))
@@ -255,7 +260,8 @@ class Solver:
     def fields_raw(self, dtype) -> tuple[np.ndarray, np.ndarray]:
         X, Y, Z = self.dims
         B = getattr(self, "slots", 1)
-        shape = (Z, Y, X) if B == 1 else (Z, Y, X, B)
+        grouped = getattr(self, "batch_layout", N.BATCH_INTERLEAVED) == N.BATCH_GROUPED
+        shape = (Z, Y, X) if B == 1 else ((B, Z, Y, X) if grouped else (Z, Y, X, B))
         p = np.zeros(shape, dtype)
         q = np.zeros(shape, dtype)
         self._c("pfdtd_get_fields_raw", N.ptr(p), N.ptr(q), p.nbytes)

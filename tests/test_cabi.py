@@ -108,6 +108,8 @@ def test_binding_constants_match_header():
              "PFDTD_BF16": native.BF16, "PFDTD_F16_DIFF": native.F16_DIFF, "PFDTD_SWEEP_AUTO": native.SWEEP_AUTO,
              "PFDTD_SWEEP_COLUMN": native.SWEEP_COLUMN, "PFDTD_SWEEP_WIDE": native.SWEEP_WIDE,
              "PFDTD_SWEEP_PAIR": native.SWEEP_PAIR,
+             "PFDTD_BATCH_INTERLEAVED": native.BATCH_INTERLEAVED,
+             "PFDTD_BATCH_GROUPED": native.BATCH_GROUPED,
              "PFDTD_OK": native.OK, "PFDTD_E_ARG": native.E_ARG, "PFDTD_E_HIP": native.E_HIP,
              "PFDTD_E_NOMEM": native.E_NOMEM, "PFDTD_E_STATE": native.E_STATE,
              "PFDTD_E_DOMAIN": native.E_DOMAIN, "PFDTD_SRC_HARD": native.SRC_HARD,
