@@ -278,6 +278,18 @@ def main() -> int:
                     "roofline_frac": round(nodes * 8 / (k16 * 1e-3) / 1e9 / PEAK_GBS, 4),
                     "speedup_vs_fp32": round(wall / wall16, 3)}
             log(f"FP16 storage: {fp16}")
+            if X % 2 == 0:
+                # binary16 difference form (P, P - P_past): the two-column sweep, 3 waves/SIMD
+                s.prepare_fields(N.F16_DIFF, 12)
+                walld, _ = timed_run(s, args.steps, args.warmup, N.F16_DIFF)
+                kd = kernel_avg_ms(s, min(args.steps, 100), N.F16_DIFF)
+                fp16["difference_form"] = {
+                    "value": round(nodes * args.steps / walld / 1e9, 2),
+                    "ms_per_step": round(walld * 1e3 / args.steps, 4),
+                    "kernel_ms": round(kd, 4),
+                    "roofline_frac": round(nodes * 8 / (kd * 1e-3) / 1e9 / PEAK_GBS, 4),
+                    "speedup_vs_fp32": round(wall / walld, 3)}
+                log(f"FP16 difference form: {fp16['difference_form']}")
 
     traffic = committed_traffic() if args.scale == 1 else None
     launch_s = 2 * kms * 1e-3

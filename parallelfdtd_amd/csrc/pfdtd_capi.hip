@@ -1156,9 +1156,10 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   rc = ensure_fields(c, precision, scale);
   if (rc) return rc;
   const bool diff = precision == PFDTD_F16_DIFF;
-  if (diff && (!fused_eligible(c) || c->scheme != 0 || c->X % 2 != 0 || c->lgB > 0))
+  if (diff && (!fused_eligible(c) || c->scheme != 0 || c->X % 2 != 0 || c->lgB > 0 ||
+               c->X * c->Y * 2 >= ((int64_t)1 << 31)))
     return fail(PFDTD_E_STATE, "the difference form runs through the two-column sweep only: forward "
-                "update, X even, no batch, no source on the slices z = 0 or Z-1");
+                "update, X even, X*Y below 2^30, no batch, no source on the slices z = 0 or Z-1");
   rc = zero_fields(c);
   if (rc) return rc;
   rc = ensure_responses(c, n_steps);

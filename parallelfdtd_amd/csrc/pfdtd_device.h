@@ -62,10 +62,11 @@ struct Fmt<b16> {
     return __uint_as_float((uint32_t)p[i].bits << 16);
   }
   static __device__ __forceinline__ uint16_t bits(float v) {
-    uint32_t x = __float_as_uint(v);
-    if ((x & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((x >> 16) | 0x40u);  // quiet NaN
-    x += 0x7FFFu + ((x >> 16) & 1u);  // round to nearest even
-    return (uint16_t)(x >> 16);
+    const uint32_t x = __float_as_uint(v);
+    const uint32_t rne = (x + 0x7FFFu + ((x >> 16) & 1u)) >> 16;  // round to nearest even
+    const uint32_t qnan = (x >> 16) | 0x40u;                       // NaN stays a (quiet) NaN
+    // a select, not a branch: a lane-divergent branch here made every rounding a masked region
+    return (uint16_t)((x & 0x7FFFFFFFu) > 0x7F800000u ? qnan : rne);
   }
   static __device__ __forceinline__ void st(b16 *p, int64_t i, float v) { p[i].bits = bits(v); }
   static __device__ __forceinline__ float val(b16 s) { return __uint_as_float((uint32_t)s.bits << 16); }

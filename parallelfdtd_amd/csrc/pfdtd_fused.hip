@@ -636,6 +636,34 @@ __device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, u
   __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(e * 2u), 0, 0);
 }
 
+// Slice access of the difference-form sweep: the lane's column pair is the vector offset (one
+// register for every row of the tile) and the row is the wave-uniform scalar offset.
+__device__ __forceinline__ uint32_t bld_pair(__amdgpu_buffer_rsrc_t r, uint32_t vcol,
+                                             uint32_t srow) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vcol, (int)srow, 0);
+}
+__device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t vcol, uint32_t srow,
+                                         uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)vcol, (int)srow, 0);
+}
+// A lane that must not store sets bit 31 of its vector offset. Slices of the two-column sweep
+// are under 2^31 bytes (the host checks it), so that offset is past every slice's range and the
+// buffer range check drops the write: masked stores need no lane-masked branch.
+constexpr uint32_t kDropShift = 0x80000000u;
+// pack_halves for values that are already exact in the storage format (rounded results): the
+// conversion is exact, so it needs no register barrier (and no copy of a value still in use).
+template <typename S>
+__device__ __forceinline__ uint16_t exact_bits(float v) {
+  if constexpr (std::is_same<S, h16>::value)
+    return __builtin_bit_cast(uint16_t, (_Float16)v);
+  else
+    return (uint16_t)(__float_as_uint(v) >> 16);
+}
+template <typename S>
+__device__ __forceinline__ uint32_t pack_exact(float a, float b) {
+  return (uint32_t)exact_bits<S>(a) | ((uint32_t)exact_bits<S>(b) << 16);
+}
+
 // Rows per wave tile and waves per block (a column of wave tiles) of the two-column sweep.
 // Measured on the 2048x1536x768 room: 4-row tiles (126 VGPRs, four waves per SIMD); 6-row tiles
 // need more than 128 VGPRs and spill. Blocks of 2 or 8 waves and z-chunks of 8 or 32 slices
@@ -643,14 +671,13 @@ __device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, u
 constexpr int kPairTY = 4;
 constexpr int kPairWY = 4;
 
-// DIFF: the difference form (PFDTD_F16_DIFF). A holds P^n and B holds D^n = P^n - P^{n-1}; the
-// past value of a node is P^n - D^n (FP32), and each level stores fl16(ret) and fl16(ret - P),
-// ret the FP32 result: P^{n+2} into Dn and D^{n+2} into Cn (both at the second level), or with
-// one level (a.levels == 1) P^{n+1} into Dn and D^{n+1} into Cn. A source sample keeps the past
-// value: D = fl16(P_new - (P_old - D_old)). The level-1 differences of the owned rows stay in
-// registers for the second level (dcur, dprev), so this variant runs at three waves per SIMD.
-template <typename S, bool DIFF>
-__global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_kernel(
+// The two-column sweep of plain 16-bit storage (P^n in A, P^{n-1} in B, P^{n+1} to Cn, P^{n+2}
+// to Dn). Per-node selects and lane-masked stores, row offsets in registers, no prefetch: on the
+// 2048x1536x768 room this form is 5 % faster than the branch-free prefetching form of the
+// difference-form kernel below (which issues every buffer access with a scalar row offset, two
+// issue cycles each, to fit its registers).
+template <typename S>
+__global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
     Fused2Ctx<S, CoefF, float> p) {
 #pragma clang fp contract(off)
   typedef float C;
@@ -685,7 +712,6 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
   const int yt = (byi * WY + wave) * TY;  // first owned row
   const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);
-  const bool two = !DIFF || a.levels >= 2;  // second time level (the difference form may take one)
 
   extern __shared__ __align__(16) unsigned char s_dyn[];
   CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
@@ -762,9 +788,6 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
   };
   constexpr uint64_t kAllRows = (1ull << NQ) - 1ull;
   uint64_t ar_win = air_window(z0 - 1);
-  Pair2 dprev[TY];  // DIFF: D^{n+1} of the owned rows at the slice the second level updates
-#pragma unroll
-  for (int o = 0; o < TY; ++o) dprev[o] = Pair2{0.f, 0.f};
 
   // P^n rows as (column x0, column x0+1) pairs of FP32 values, P^{n+1} likewise, packed
   // position|material of both columns in one register per row.
@@ -821,7 +844,6 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
     for (int r = 0; r < NA; ++r) ap[r] = Pair2{lo_half<S>(ua[r]), hi_half<S>(ua[r])};
 
     // ---- first time level: P^{n+1}[k]
-    Pair2 dcur[TY];  // DIFF: D^{n+1}[k] of the owned rows
     auto level1 = [&](auto table) {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) {
@@ -836,19 +858,13 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
         s1 = s1 + ac[r - 1].y;
         s1 = s1 + lane_next(ac[r].x);        // x+1: the next lane's lower column
         s1 = s1 + ac[r].x;                   // x-1
-        const C pp0 = DIFF ? ac[r].x - lo_half<S>(pu[j]) : lo_half<S>(pu[j]);
-        const C pp1 = DIFF ? ac[r].y - hi_half<S>(pu[j]) : hi_half<S>(pu[j]);
+        const C pp0 = lo_half<S>(pu[j]);
+        const C pp1 = hi_half<S>(pu[j]);
         const C w0 = node(table, bc[j] & 0xFFFFu, ac[r].x, s0, pp0);
         const C w1 = node(table, bc[j] >> 16, ac[r].y, s1, pp1);
         const bool row = upd1 && ((ru >> r) & 1u);
         q2[j].x = (row && cu0) ? Fmt<S>::round(w0) : (C)0;
         q2[j].y = (row && cu1) ? Fmt<S>::round(w1) : (C)0;
-        if constexpr (DIFF) {
-          if (j >= 1 && j <= TY) {
-            dcur[j - 1].x = (row && cu0) ? Fmt<S>::round(w0 - ac[r].x) : (C)0;
-            dcur[j - 1].y = (row && cu1) ? Fmt<S>::round(w1 - ac[r].y) : (C)0;
-          }
-        }
       }
     };
     if (!air_fast || (!air1 && any_nonair(bc))) {
@@ -895,39 +911,22 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
             const C v = hard ? Fmt<S>::round(smp) : Fmt<S>::round(old + smp);
             q2[j].x = hi ? q2[j].x : v;
             q2[j].y = hi ? v : q2[j].y;
-            if constexpr (DIFF) {
-              if (j >= 1 && j <= TY) {  // owned rows: keep the past value
-                const C past = old - pick(dcur[j - 1], hi);
-                const C d = Fmt<S>::round(v - past);
-                dcur[j - 1].x = hi ? dcur[j - 1].x : d;
-                dcur[j - 1].y = hi ? d : dcur[j - 1].y;
-              }
-            }
           }
       }
     }
 
-    // ---- store the owned part of P^{n+1}[k] (the difference form with one level: P^{n+1}
-    // into Dn and D^{n+1} into Cn; with two levels it stores at the second level only)
-    if (k >= z0 && k < z1 && (!DIFF || !two)) {
-      const auto rC = slice_rsrc((DIFF ? gD : gC) + (size_t)k * XY, sbytes);
+    // ---- store the owned part of P^{n+1}[k]
+    if (k >= z0 && k < z1) {
+      const auto rC = slice_rsrc(gC + (size_t)k * XY, sbytes);
 #pragma unroll
       for (int o = 0; o < TY; ++o)
         if (((rs >> o) & 1u) && owned_lane)
           bst_pair(rC, ro[o + 2], pack_halves<S>(q2[o + 1].x, q2[o + 1].y));
-      if constexpr (DIFF) {
-        const auto rDd = slice_rsrc(gC + (size_t)k * XY, sbytes);
-#pragma unroll
-        for (int o = 0; o < TY; ++o)
-          if (((rs >> o) & 1u) && owned_lane)
-            bst_pair(rDd, ro[o + 2], pack_halves<S>(dcur[o].x, dcur[o].y));
-      }
     }
 
     // ---- second time level: P^{n+2}[k-1] on the owned rows
-    if (two && k - 1 >= z0) {
+    if (k - 1 >= z0) {
       const auto rD = slice_rsrc(gD + (size_t)(k - 1) * XY, sbytes);
-      const auto rCd = slice_rsrc(gC + (size_t)(k - 1) * XY, sbytes);
       auto level2 = [&](auto table) {
 #pragma unroll
         for (int o = 0; o < TY; ++o) {
@@ -942,16 +941,9 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
           s1 = s1 + q1[j - 1].y;
           s1 = s1 + lane_next(q1[j].x);
           s1 = s1 + q1[j].x;
-          const C pp0 = DIFF ? q1[j].x - dprev[o].x : am[o + 2].x;
-          const C pp1 = DIFF ? q1[j].y - dprev[o].y : am[o + 2].y;
-          C v0 = node(table, bp[j] & 0xFFFFu, q1[j].x, s0, pp0);
-          C v1 = node(table, bp[j] >> 16, q1[j].y, s1, pp1);
+          C v0 = node(table, bp[j] & 0xFFFFu, q1[j].x, s0, am[o + 2].x);
+          C v1 = node(table, bp[j] >> 16, q1[j].y, s1, am[o + 2].y);
           const bool row = (ru >> (o + 2)) & 1u;
-          if constexpr (DIFF) {
-            const C e0 = (row && cu0) ? Fmt<S>::round(v0 - q1[j].x) : (C)0;
-            const C e1 = (row && cu1) ? Fmt<S>::round(v1 - q1[j].y) : (C)0;
-            if (((rs >> o) & 1u) && owned_lane) bst_pair(rCd, ro[o + 2], pack_halves<S>(e0, e1));
-          }
           v0 = (row && cu0) ? v0 : (C)0;
           v1 = (row && cu1) ? v1 : (C)0;
           if (((rs >> o) & 1u) && owned_lane) bst_pair(rD, ro[o + 2], pack_halves<S>(v0, v1));
@@ -962,10 +954,6 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
       } else {
         level2(std::false_type());
       }
-    }
-    if constexpr (DIFF) {
-#pragma unroll
-      for (int o = 0; o < TY; ++o) dprev[o] = dcur[o];
     }
   };
 
@@ -1005,6 +993,426 @@ __global__ __launch_bounds__(64 * kPairWY, DIFF ? 3 : 4) void srl_update2_pair_k
     body(A1, A2, A0, Q1, Q2, Q0, B1, B2, k);
     if (++k > k1) break;
     body(A2, A0, A1, Q2, Q0, Q1, B2, B0, k);
+    if (++k > k1) break;
+  }
+}
+
+// The difference form (PFDTD_F16_DIFF). A holds P^n and B holds D^n = P^n - P^{n-1}; the
+// past value of a node is P^n - D^n (FP32), and each level stores fl16(ret) and fl16(ret - P),
+// ret the FP32 result: P^{n+2} into Dn and D^{n+2} into Cn (both at the second level), or with
+// one level (a.levels == 1) P^{n+1} into Dn and D^{n+1} into Cn. A source sample keeps the past
+// value: D = fl16(P_new - (P_old - D_old)). The level-1 differences of the owned rows stay in
+// registers for the second level (dcur, dprev), so this variant runs at three waves per SIMD.
+// It is written branch-free: every lane evaluates every node and edge waves zero the nodes
+// outside the updated range afterwards, masked stores drop through the buffer range check, the
+// loads of the next slice are issued as soon as the first level has consumed its own, and the
+// row offsets are scalar (one vector register for the lane's columns); at three waves per
+// SIMD this is 8 % faster than the selects and lane-masked stores of the plain kernel above.
+__global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
+    Fused2Ctx<h16, CoefF, float> p) {
+#pragma clang fp contract(off)
+  typedef h16 S;
+  typedef float C;
+  constexpr int TY = kPairTY, WY = kPairWY;
+  constexpr int NA = TY + 4;  // P^n rows (2-voxel y halo)
+  constexpr int NQ = TY + 2;  // P^{n+1} rows (1-voxel y halo)
+  __shared__ int s_src[WY][kFListCap];
+  __shared__ int s_rec[WY][kFListCap];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  Fused2Args<S, CoefF, float> a = p.a;
+  const int X = p.X, Y = p.Y, Z = p.Z, zc = p.zc;
+  // XCD-aware tile order, as in srl_update2_kernel (grouped-batch slots innermost)
+  const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+  const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned full = total & ~7u;
+  unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+  const int slot = (int)(tile % (unsigned)a.n_slots);
+  tile /= (unsigned)a.n_slots;
+  if (slot) {
+    const int64_t off = (int64_t)slot * a.slot_elems;
+    a.A += off;
+    a.B += off;
+    a.Cn += off;
+    a.Dn += off;
+  }
+  const int xt = (int)(tile % nx);
+  const int byi = (int)((tile / nx) % ny);
+  const int bzi = (int)(tile / (nx * ny));
+  const int xb = xt * kPO;               // first owned column
+  const int x0 = xb - 2 + 2 * lane;      // this lane's columns: x0 and x0 + 1
+  const int yt = (byi * WY + wave) * TY;  // first owned row
+  const int z0 = 1 + bzi * zc;
+  const int z1 = min(z0 + zc, Z - 1);
+  const bool two = a.levels >= 2;  // the second time level (the odd last step takes one)
+
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
+  const unsigned ncol = (unsigned)a.ncol;
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WY)
+    s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
+  const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kPO + 1, yt - 1, yt + TY + 1,
+                                z0 - 1, z1 + 1, s_src[wave], lane, a.src_slot, slot);
+  const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kPO, yt, yt + TY, z0, z1,
+                                s_rec[wave], lane, a.rec_slot, slot);
+  __syncthreads();
+
+  const size_t XY = (size_t)X * (size_t)Y;
+  const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^31
+  const int xc = min(max(x0, 0), X - 2);  // an even column of the grid
+  const uint32_t vcol = (uint32_t)xc * 2u;  // byte offset of the lane's pair within a row
+  // byte offset of tile row r (rows outside the grid read the nearest grid row, never stored)
+  auto srow = [&](int r) -> uint32_t {
+    return (uint32_t)(min(max(yt - 2 + r, 0), Y - 1) * X) * 2u;
+  };
+  uint32_t row_upd = 0;
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    const int y = yt - 2 + r;
+    if (y > 0 && y < Y - 1) row_upd |= 1u << r;
+  }
+  // whether this lane's two columns are updated (not 0 or X-1): evaluated where needed, on the
+  // edge waves only, from a copy of x0 the compiler cannot hoist, so that no register holds it
+  // across the sweep
+  auto col_upd = [&]() {
+    int xl = x0;
+    asm volatile("" : "+v"(xl));
+    return std::pair<bool, bool>(xl > 0 && xl < X - 1, xl + 1 > 0 && xl + 1 < X - 1);
+  };
+  const bool owned_lane = lane >= 1 && lane <= 62 && x0 < X;
+  const uint32_t vst = vcol | (owned_lane ? 0u : kDropShift);  // halo lanes, lanes past the grid
+  const bool rows_in = yt + TY <= Y;  // every owned row lies in the grid
+  // every column of the wave (xb-2 .. xb+125) lies in 1..X-2: no lane needs a column select
+  const bool x_inner = xb - 2 >= 1 && xb + 2 * 64 - 3 <= X - 2;
+  const double inv_scale = ldexp(1.0, -a.scale_exp);
+  const C scale = (C)ldexp(1.0, a.scale_exp);
+  const CoefF air = a.air;
+  const C lam2 = a.lam2;
+  // Plain air with the entry (inv, omb, tk) = (1, 1, 0) -- every air voxel when lambda^2 rounds
+  // to 1/3 in FP32, since 2 - 6 lambda^2 is then exactly 0 -- takes
+  //   fma(0, p, lambda^2 s) - p_past,
+  // the same bits as inv*((tk*p + lambda^2 s) - omb*p_past): the products by 1 are exact, and the
+  // product 0*p is exact, so the fused add rounds exactly as the separate one (signed zeros
+  // included). Otherwise every slice takes the table.
+  const bool air_fast = a.air_uniform != 0 && air.inv == 1.f && air.omb == 1.f && air.tk == 0.f;
+  auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
+    if constexpr (decltype(table)::value) {
+      const unsigned pb = pm & 0xFFu;
+      const CoefF e = s_coef[min(pb & 0x7Fu, 6u) * ncol + (pm >> 8)];
+      const C aa = (C)(pb >> 7) * e.inv;
+      return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
+    } else {
+      return __builtin_fmaf(0.f, pc, lam2 * s) - past;
+    }
+  };
+  auto any_nonair = [&](const uint32_t (&b)[NQ]) -> bool {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) m |= ((b[i] & 0xFFu) ^ 0x86u) | (((b[i] >> 16) & 0xFFu) ^ 0x86u);
+    return __ballot(m != 0u) != 0;
+  };
+  const S *__restrict__ gA = a.A;
+  const S *__restrict__ gB = a.B;
+  S *__restrict__ gC = a.Cn;
+  S *__restrict__ gD = a.Dn;
+  const uint16_t *__restrict__ gpm = a.pm;
+  typedef const __attribute__((address_space(4))) uint32_t *ConstU32;
+  const uint32_t *__restrict__ gar = a.air_rows;
+  const int nyw = Y / 32 + 2;
+  const size_t ar_slice = (size_t)((X + kPO - 1) / kPO) * (size_t)nyw;
+  const bool past_grid = yt >= Y || xb >= X;
+  const uint32_t *ar_tile = gar && !past_grid ? gar + (size_t)xt * nyw + (yt >> 5) : nullptr;
+  auto air_window = [&](int k) -> uint64_t {
+    if (!gar || k < 1 || k > Z - 2) return 0;
+    if (past_grid) return ~0ull;
+    const ConstU32 w = (ConstU32)(ar_tile + (size_t)k * ar_slice);
+    return (((uint64_t)w[1] << 32) | w[0]) >> (yt & 31);
+  };
+  constexpr uint64_t kAllRows = (1ull << NQ) - 1ull;
+  Pair2 dprev[TY];  // D^{n+1} of the owned rows at the slice the second level updates
+#pragma unroll
+  for (int o = 0; o < TY; ++o) dprev[o] = Pair2{0.f, 0.f};
+
+  // P^n rows as (column x0, column x0+1) pairs of FP32 values, P^{n+1} likewise, packed
+  // position|material of both columns in one register per row. The first level at slice k
+  // reads P^n[k+1] (fetch_a), D^n[k] (fetch_b) and the packed plane at k (fetch_pm: constant
+  // air values when the air-row bits say every row is plain air), as raw 32-bit registers.
+  auto fetch_a = [&](int k, uint32_t (&ua)[NA]) {
+    if (k + 1 <= Z - 1) {
+      const auto rA = slice_rsrc(gA + (size_t)(k + 1) * XY, sbytes);
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ua[r] = bld_pair(rA, vcol, srow(r));
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ua[r] = 0;
+    }
+  };
+  auto fetch_b = [&](int k, uint32_t (&ub)[NQ]) {
+    const bool upd = k >= 1 && k <= Z - 2;
+    if (upd) {
+      const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) ub[j] = bld_pair(rB, vcol, srow(j + 1));
+    } else {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) ub[j] = 0;
+    }
+  };
+  auto fetch_pm = [&](int k, uint32_t (&um)[NQ], bool &uair, uint64_t win) {
+    const bool upd = k >= 1 && k <= Z - 2;
+    uair = upd && (win & kAllRows) == kAllRows;
+    if (uair) {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) um[j] = 0x00860086u;
+    } else if (upd) {
+      const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) um[j] = bld_pair(rPM, vcol, srow(j + 1));
+    } else {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) um[j] = 0;
+    }
+  };
+  // Software pipeline across slices: the loads the first level of slice k+1 reads (P^n[k+2],
+  // D^n[k+1], the packed plane at k+1) are issued as soon as the first level of slice k has
+  // consumed its own, into the same registers (ua, pu: raw 32-bit pairs; the packed plane into
+  // the third array of the rotation), so they are in flight while this wave runs its second
+  // level and stores. air_k: whether slice k's rows are all plain air.
+  uint32_t ua[NA], pu[NQ];
+  bool air_k = false;
+  auto body = [&](Pair2 (&am)[NA], Pair2 (&ac)[NA], Pair2 (&ap)[NA], Pair2 (&q0)[NQ],
+                  Pair2 (&q1)[NQ], Pair2 (&q2)[NQ], uint32_t (&bp)[NQ], uint32_t (&bc)[NQ],
+                  uint32_t (&bn)[NQ], int k) {
+    uint32_t ru = __builtin_amdgcn_readfirstlane(row_upd);
+    asm volatile("" : "+s"(ru));
+    __builtin_amdgcn_s_barrier();
+    const bool upd1 = k >= 1 && k <= Z - 2;
+    const bool air1 = air_k;
+    const uint64_t win_next = air_window(k + 1);  // for the loads of slice k+1
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) ap[r] = Pair2{lo_half<S>(ua[r]), hi_half<S>(ua[r])};
+
+    // ---- first time level: P^{n+1}[k]
+    Pair2 dcur[TY];  // D^{n+1}[k] of the owned rows
+    auto level1 = [&](auto table) {
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const int r = j + 1;
+        C s0 = ap[r].x + am[r].x;            // z+1, z-1
+        s0 = s0 + ac[r + 1].x;               // y+1
+        s0 = s0 + ac[r - 1].x;               // y-1
+        s0 = s0 + ac[r].y;                   // x+1: the pair's other column
+        s0 = s0 + lane_prev(ac[r].y);        // x-1: the previous lane's upper column
+        C s1 = ap[r].y + am[r].y;
+        s1 = s1 + ac[r + 1].y;
+        s1 = s1 + ac[r - 1].y;
+        s1 = s1 + lane_next(ac[r].x);        // x+1: the next lane's lower column
+        s1 = s1 + ac[r].x;                   // x-1
+        const C pp0 = ac[r].x - lo_half<S>(pu[j]);  // the past value P^n - D^n
+        const C pp1 = ac[r].y - hi_half<S>(pu[j]);
+        // every lane evaluates every node (all operands come from in-range loads or read 0);
+        // the nodes outside the updated range are zeroed afterwards
+        const C w0 = node(table, bc[j] & 0xFFFFu, ac[r].x, s0, pp0);
+        const C w1 = node(table, bc[j] >> 16, ac[r].y, s1, pp1);
+        q2[j].x = Fmt<S>::round(w0);
+        q2[j].y = Fmt<S>::round(w1);
+        if (j >= 1 && j <= TY) {
+          dcur[j - 1].x = Fmt<S>::round(w0 - ac[r].x);
+          dcur[j - 1].y = Fmt<S>::round(w1 - ac[r].y);
+        }
+      }
+    };
+    if (!air_fast || (!air1 && any_nonair(bc))) {
+      level1(std::true_type());
+    } else {
+      level1(std::false_type());
+    }
+    // Zero the nodes outside the updated range: the rows and slices the grid does not update,
+    // and the columns 0 and X-1. Only waves at the grid's edges take this branch. Selects inside
+    // the node loop (`row && cu ? round(w) : 0`) made every node a lane-masked branch of its
+    // own -- exec-mask saves and restores per node, and DPP shifts that no longer fold into
+    // their adds -- or, written with bitwise masks, cost five instructions per select.
+    if (!(upd1 && x_inner && ((ru >> 1) & kAllRows) == kAllRows)) {
+      const auto [cu0, cu1] = col_upd();
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const bool row = upd1 && ((ru >> (j + 1)) & 1u);
+        const bool m0 = row && cu0, m1 = row && cu1;
+        q2[j].x = m0 ? q2[j].x : (C)0;
+        q2[j].y = m1 ? q2[j].y : (C)0;
+        if (j >= 1 && j <= TY) {
+          dcur[j - 1].x = m0 ? dcur[j - 1].x : (C)0;
+          dcur[j - 1].y = m1 ? dcur[j - 1].y : (C)0;
+        }
+      }
+    }
+    // the first level is done with ua / pu: issue slice k+1's loads into them
+    __builtin_amdgcn_sched_barrier(0);
+    if (k + 1 <= z1) {
+      fetch_a(k + 1, ua);
+      fetch_b(k + 1, pu);
+      fetch_pm(k + 1, bn, air_k, win_next);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+
+    // ---- receivers (before injection) and sources on P^{n+1}[k]
+    if (nrec != 0) {
+      const int cnt = nrec < 0 ? a.n_rec : nrec;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nrec < 0 ? t : s_rec[wave][t];
+        const int32_t *q = a.rec_xyz + 3 * id;
+        if (q[2] != k || (q[0] & ~1) != x0) continue;
+        if (nrec < 0 && (!in_box(q, xb, xb + kPO, yt, yt + TY, z0, z1) ||
+                         (a.rec_slot && a.rec_slot[id] != slot)))
+          continue;
+        const bool hi = q[0] & 1;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          if (q[1] == yt - 1 + j)
+            a.resp[(int64_t)id * a.resp_steps + a.gather_step] =
+                (double)pick(q2[j], hi) * inv_scale;
+      }
+    }
+    if (nsrc != 0 && a.inject_step >= 0) {
+      const int cnt = nsrc < 0 ? a.n_src : nsrc;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nsrc < 0 ? t : s_src[wave][t];
+        const int32_t *q = a.src_xyz + 3 * id;
+        if (q[2] != k || (q[0] & ~1) != x0) continue;
+        if (nsrc < 0 && a.src_slot && a.src_slot[id] != slot) continue;
+        C smp = (C)0;
+        if (a.inject_step < a.n_samples) smp = a.sig_f[(int64_t)id * a.n_samples + a.inject_step];
+        smp = smp * scale;
+        const bool hard = a.src_kind[id] == 0 || a.soft_overwrite;
+        const bool hi = q[0] & 1;
+        // value selects only: a reference chosen at run time would put the queue in memory
+#pragma unroll
+        for (int j = 0; j < NQ; ++j)
+          if (q[1] == yt - 1 + j) {
+            const C old = pick(q2[j], hi);
+            const C v = hard ? Fmt<S>::round(smp) : Fmt<S>::round(old + smp);
+            q2[j].x = hi ? q2[j].x : v;
+            q2[j].y = hi ? v : q2[j].y;
+            if (j >= 1 && j <= TY) {  // owned rows: keep the past value
+              const C past = old - pick(dcur[j - 1], hi);
+              const C d = Fmt<S>::round(v - past);
+              dcur[j - 1].x = hi ? dcur[j - 1].x : d;
+              dcur[j - 1].y = hi ? d : dcur[j - 1].y;
+            }
+          }
+      }
+    }
+
+    // ---- one level only (the odd last step): P^{n+1}[k] into Dn and D^{n+1}[k] into Cn; with
+    // two levels the second level stores
+    if (k >= z0 && k < z1 && !two) {
+      const auto rC = slice_rsrc(gD + (size_t)k * XY, sbytes);
+      const auto rDd = slice_rsrc(gC + (size_t)k * XY, sbytes);
+      auto store_rows = [&](int nrows) {
+#pragma unroll
+        for (int o = 0; o < TY; ++o) {
+          if (o >= nrows) break;
+          bst_pair(rC, vst, srow(o + 2), pack_exact<S>(q2[o + 1].x, q2[o + 1].y));
+          bst_pair(rDd, vst, srow(o + 2), pack_exact<S>(dcur[o].x, dcur[o].y));
+        }
+      };
+      if (rows_in)
+        store_rows(TY);
+      else  // the last row of wave tiles: rows past the grid skip their stores
+        store_rows(Y - yt);
+    }
+
+    // ---- second time level: P^{n+2}[k-1] on the owned rows
+    if (two && k - 1 >= z0) {
+      const auto rD = slice_rsrc(gD + (size_t)(k - 1) * XY, sbytes);
+      const auto rCd = slice_rsrc(gC + (size_t)(k - 1) * XY, sbytes);
+      constexpr uint32_t kOwnedRows = (1u << TY) - 1u;
+      const bool inner2 = x_inner && ((ru >> 2) & kOwnedRows) == kOwnedRows;
+      auto level2 = [&](auto table) {
+#pragma unroll
+        for (int o = 0; o < TY; ++o) {
+          const int j = o + 1;
+          C s0 = q2[j].x + q0[j].x;          // z+1, z-1
+          s0 = s0 + q1[j + 1].x;             // y+1
+          s0 = s0 + q1[j - 1].x;             // y-1
+          s0 = s0 + q1[j].y;                 // x+1
+          s0 = s0 + lane_prev(q1[j].y);      // x-1
+          C s1 = q2[j].y + q0[j].y;
+          s1 = s1 + q1[j + 1].y;
+          s1 = s1 + q1[j - 1].y;
+          s1 = s1 + lane_next(q1[j].x);
+          s1 = s1 + q1[j].x;
+          const C pp0 = q1[j].x - dprev[o].x;  // the past value P^{n+1} - D^{n+1}
+          const C pp1 = q1[j].y - dprev[o].y;
+          C v0 = node(table, bp[j] & 0xFFFFu, q1[j].x, s0, pp0);
+          C v1 = node(table, bp[j] >> 16, q1[j].y, s1, pp1);
+          C e0 = Fmt<S>::round(v0 - q1[j].x);
+          C e1 = Fmt<S>::round(v1 - q1[j].y);
+          uint32_t vs = vst;
+          if (!inner2) {  // edge waves: zero the nodes outside the updated range
+            const auto [cu0, cu1] = col_upd();
+            const bool row = (ru >> (o + 2)) & 1u;
+            const bool m0 = row && cu0, m1 = row && cu1;
+            v0 = m0 ? v0 : (C)0;
+            v1 = m1 ? v1 : (C)0;
+            e0 = m0 ? e0 : (C)0;
+            e1 = m1 ? e1 : (C)0;
+            if (yt + o >= Y) vs |= kDropShift;  // a row past the grid: drop its stores
+          }
+          bst_pair(rCd, vs, srow(o + 2), pack_exact<S>(e0, e1));
+          bst_pair(rD, vs, srow(o + 2), pack_halves<S>(v0, v1));
+        }
+      };
+      if (!air_fast || any_nonair(bp)) {
+        level2(std::true_type());
+      } else {
+        level2(std::false_type());
+      }
+    }
+#pragma unroll
+    for (int o = 0; o < TY; ++o) dprev[o] = dcur[o];
+  };
+
+  Pair2 A0[NA], A1[NA], A2[NA];
+  Pair2 Q0[NQ], Q1[NQ], Q2[NQ];
+  uint32_t B0[NQ], B1[NQ], B2[NQ];
+  const int k0 = z0 - 1;
+  const int k1 = z1;
+  {
+    fetch_a(k0, ua);  // the first body's slice
+    fetch_b(k0, pu);
+    fetch_pm(k0, B1, air_k, air_window(k0));
+    const auto rK = slice_rsrc(gA + (size_t)k0 * XY, sbytes);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) {
+      const uint32_t u = bld_pair(rK, vcol, srow(r));
+      A1[r] = Pair2{lo_half<S>(u), hi_half<S>(u)};
+    }
+    if (k0 >= 1) {
+      const auto rKm = slice_rsrc(gA + (size_t)(k0 - 1) * XY, sbytes);
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const uint32_t u = bld_pair(rKm, vcol, srow(r));
+        A0[r] = Pair2{lo_half<S>(u), hi_half<S>(u)};
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) A0[r] = Pair2{0.f, 0.f};
+    }
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      Q0[j] = Pair2{0.f, 0.f};
+      Q1[j] = Pair2{0.f, 0.f};
+      B0[j] = 0;
+    }
+  }
+  for (int k = k0;;) {
+    body(A0, A1, A2, Q0, Q1, Q2, B0, B1, B2, k);
+    if (++k > k1) break;
+    body(A1, A2, A0, Q1, Q2, Q0, B1, B2, B0, k);
+    if (++k > k1) break;
+    body(A2, A0, A1, Q2, Q0, Q1, B2, B0, B1, k);
     if (++k > k1) break;
   }
 }
@@ -1069,8 +1477,9 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   g.scheme = scheme;
   g.wide = !pool && precision == 0 && scheme == 0 &&
            (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16));
-  // 16-bit storage, forward scheme, X even: two columns per lane
+  // 16-bit storage, forward scheme, X even, slices under 2^31 bytes: two columns per lane
   g.pair = !pool && precision >= 2 && scheme == 0 && X % 2 == 0 &&
+           (int64_t)X * Y * 2 < (int64_t)kDropShift &&
            (shape == PFDTD_SWEEP_AUTO || shape == PFDTD_SWEEP_PAIR);
   if (g.pair) {
     const int rows = kPairTY * kPairWY;
@@ -1113,11 +1522,11 @@ static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a
     if (g.pair) {
       if constexpr (std::is_same<S, h16>::value) {
         if (a.diff) {
-          hipLaunchKernelGGL((srl_update2_pair_kernel<S, true>), g.grid, g.block, lds, stream, p);
+          hipLaunchKernelGGL(srl_update2_pair_diff_kernel, g.grid, g.block, lds, stream, p);
           return hipGetLastError();
         }
       }
-      hipLaunchKernelGGL((srl_update2_pair_kernel<S, false>), g.grid, g.block, lds, stream, p);
+      hipLaunchKernelGGL((srl_update2_pair_kernel<S>), g.grid, g.block, lds, stream, p);
       return hipGetLastError();
     }
   }
