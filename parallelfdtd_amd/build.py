@@ -21,7 +21,9 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# gfx950 only; packed-FP32 instruction selection disabled ("Build").
+# gfx950 only. Packed-FP32 instruction selection is disabled: the compiler would otherwise pair
+# ordinary float code into v_pk_* instructions, some forms of which have given wrong results
+# on this part.
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
