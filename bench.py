@@ -9,7 +9,8 @@ Workload (one "step" = one leapfrog time step of the whole room):
 Prints one JSON line:
   value     = Gvoxel-updates/s of the FP32 run (all nodes, shell included), host wall clock
               over exactly --steps steps after --warmup untimed steps;
-  fp16      = the same with binary16 field storage (power-of-two scaled);
+  fp16      = the same with binary16 field storage (power-of-two scaled), and beside it the
+              binary16 difference form (P, P - P_past) on rooms of even X;
   roofline  = algorithmic bytes per update-kernel launch (14 B/node FP32) / average update
               kernel duration measured with HIP events on the library's stream, vs 8 TB/s;
               `traffic` = measured HBM bytes per launch from the committed rocprofv3 counter
@@ -210,6 +211,8 @@ def main() -> int:
     ap.add_argument("--fp32-only", action="store_true", help="only the FP32 run (profiling)")
     ap.add_argument("--fp16-only", action="store_true",
                     help="only the FP16-storage run (profiling); prints no bench line")
+    ap.add_argument("--f16diff-only", action="store_true",
+                    help="only the FP16 difference-form run (profiling); prints no bench line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 4 GiB device copy (counter calibration for rocprofv3)")
     ap.add_argument("--bw-probe", action="store_true",
@@ -253,11 +256,13 @@ def main() -> int:
         s.load_scene(sc)
         log(f"domain classified on GPU in {time.perf_counter() - t0:.1f} s")
 
-        if args.fp16_only:
-            s.prepare_fields(N.F16, 12)
-            wall16, _ = timed_run(s, args.steps, args.warmup, N.F16)
-            k16 = kernel_avg_ms(s, min(args.steps, 100), N.F16)
-            log(f"FP16 storage: {nodes * args.steps / wall16 / 1e9:.1f} Gvox/s, "
+        if args.fp16_only or args.f16diff_only:
+            prec, name = (N.F16, "FP16 storage") if args.fp16_only else (N.F16_DIFF,
+                                                                         "FP16 difference form")
+            s.prepare_fields(prec, 12)
+            wall16, _ = timed_run(s, args.steps, args.warmup, prec)
+            k16 = kernel_avg_ms(s, min(args.steps, 100), prec)
+            log(f"{name}: {nodes * args.steps / wall16 / 1e9:.1f} Gvox/s, "
                 f"update {k16:.3f} ms/step")
             return 0
         wall, dev_ms = timed_run(s, args.steps, args.warmup, N.F32)
