@@ -635,17 +635,14 @@ __device__ __forceinline__ uint32_t bld_pair(__amdgpu_buffer_rsrc_t r, uint32_t 
 __device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t e, uint32_t v) {
   __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)(e * 2u), 0, 0);
 }
+// The same at a byte offset (the difference-form sweep keeps byte offsets, one per tile row).
+__device__ __forceinline__ uint32_t bld_pair_at(__amdgpu_buffer_rsrc_t r, uint32_t b) {
+  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)b, 0, 0);
+}
+__device__ __forceinline__ void bst_pair_at(__amdgpu_buffer_rsrc_t r, uint32_t b, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)b, 0, 0);
+}
 
-// Slice access of the difference-form sweep: the lane's column pair is the vector offset (one
-// register for every row of the tile) and the row is the wave-uniform scalar offset.
-__device__ __forceinline__ uint32_t bld_pair(__amdgpu_buffer_rsrc_t r, uint32_t vcol,
-                                             uint32_t srow) {
-  return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vcol, (int)srow, 0);
-}
-__device__ __forceinline__ void bst_pair(__amdgpu_buffer_rsrc_t r, uint32_t vcol, uint32_t srow,
-                                         uint32_t v) {
-  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)vcol, (int)srow, 0);
-}
 // A lane that must not store sets bit 31 of its vector offset. Slices of the two-column sweep
 // are under 2^31 bytes (the host checks it), so that offset is past every slice's range and the
 // buffer range check drops the write: masked stores need no lane-masked branch.
@@ -1006,8 +1003,9 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
 // It is written branch-free: every lane evaluates every node and edge waves zero the nodes
 // outside the updated range afterwards, masked stores drop through the buffer range check, the
 // loads of the next slice are issued as soon as the first level has consumed its own, and the
-// row offsets are scalar (one vector register for the lane's columns); at three waves per
-// SIMD this is 8 % faster than the selects and lane-masked stores of the plain kernel above.
+// row offsets are vector registers (with scalar row offsets every buffer access takes two issue
+// cycles; 1.4 % slower). At three waves per SIMD this is 9 % faster than the selects and
+// lane-masked stores of the plain kernel above.
 __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     Fused2Ctx<h16, CoefF, float> p) {
 #pragma clang fp contract(off)
@@ -1080,7 +1078,10 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     return std::pair<bool, bool>(xl > 0 && xl < X - 1, xl + 1 > 0 && xl + 1 < X - 1);
   };
   const bool owned_lane = lane >= 1 && lane <= 62 && x0 < X;
-  const uint32_t vst = vcol | (owned_lane ? 0u : kDropShift);  // halo lanes, lanes past the grid
+  const uint32_t vst = owned_lane ? 0u : kDropShift;  // halo lanes, lanes past the grid
+  uint32_t rob[NA];  // byte offset of the lane's pair in tile row r
+#pragma unroll
+  for (int r = 0; r < NA; ++r) rob[r] = srow(r) + vcol;
   const bool rows_in = yt + TY <= Y;  // every owned row lies in the grid
   // every column of the wave (xb-2 .. xb+125) lies in 1..X-2: no lane needs a column select
   const bool x_inner = xb - 2 >= 1 && xb + 2 * 64 - 3 <= X - 2;
@@ -1141,7 +1142,7 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     if (k + 1 <= Z - 1) {
       const auto rA = slice_rsrc(gA + (size_t)(k + 1) * XY, sbytes);
 #pragma unroll
-      for (int r = 0; r < NA; ++r) ua[r] = bld_pair(rA, vcol, srow(r));
+      for (int r = 0; r < NA; ++r) ua[r] = bld_pair_at(rA, rob[r]);
     } else {
 #pragma unroll
       for (int r = 0; r < NA; ++r) ua[r] = 0;
@@ -1152,7 +1153,7 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     if (upd) {
       const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) ub[j] = bld_pair(rB, vcol, srow(j + 1));
+      for (int j = 0; j < NQ; ++j) ub[j] = bld_pair_at(rB, rob[j + 1]);
     } else {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) ub[j] = 0;
@@ -1167,7 +1168,7 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     } else if (upd) {
       const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) um[j] = bld_pair(rPM, vcol, srow(j + 1));
+      for (int j = 0; j < NQ; ++j) um[j] = bld_pair_at(rPM, rob[j + 1]);
     } else {
 #pragma unroll
       for (int j = 0; j < NQ; ++j) um[j] = 0;
@@ -1313,8 +1314,8 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
 #pragma unroll
         for (int o = 0; o < TY; ++o) {
           if (o >= nrows) break;
-          bst_pair(rC, vst, srow(o + 2), pack_exact<S>(q2[o + 1].x, q2[o + 1].y));
-          bst_pair(rDd, vst, srow(o + 2), pack_exact<S>(dcur[o].x, dcur[o].y));
+          bst_pair_at(rC, rob[o + 2] | vst, pack_exact<S>(q2[o + 1].x, q2[o + 1].y));
+          bst_pair_at(rDd, rob[o + 2] | vst, pack_exact<S>(dcur[o].x, dcur[o].y));
         }
       };
       if (rows_in)
@@ -1360,8 +1361,8 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
             e1 = m1 ? e1 : (C)0;
             if (yt + o >= Y) vs |= kDropShift;  // a row past the grid: drop its stores
           }
-          bst_pair(rCd, vs, srow(o + 2), pack_exact<S>(e0, e1));
-          bst_pair(rD, vs, srow(o + 2), pack_halves<S>(v0, v1));
+          bst_pair_at(rCd, rob[o + 2] | vs, pack_exact<S>(e0, e1));
+          bst_pair_at(rD, rob[o + 2] | vs, pack_halves<S>(v0, v1));
         }
       };
       if (!air_fast || any_nonair(bp)) {
@@ -1386,14 +1387,14 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     const auto rK = slice_rsrc(gA + (size_t)k0 * XY, sbytes);
 #pragma unroll
     for (int r = 0; r < NA; ++r) {
-      const uint32_t u = bld_pair(rK, vcol, srow(r));
+      const uint32_t u = bld_pair_at(rK, rob[r]);
       A1[r] = Pair2{lo_half<S>(u), hi_half<S>(u)};
     }
     if (k0 >= 1) {
       const auto rKm = slice_rsrc(gA + (size_t)(k0 - 1) * XY, sbytes);
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
-        const uint32_t u = bld_pair(rKm, vcol, srow(r));
+        const uint32_t u = bld_pair_at(rKm, rob[r]);
         A0[r] = Pair2{lo_half<S>(u), hi_half<S>(u)};
       }
     } else {
