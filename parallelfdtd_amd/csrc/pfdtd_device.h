@@ -195,6 +195,30 @@ __device__ __forceinline__ C centred_node(unsigned pb, unsigned mb, C pc, C zp, 
   return t * e.inv;
 }
 
+// centred_node for an air voxel (position byte 0x80: inside, no DIR or SIGN bit), whatever its
+// material byte: the entry is `air`, the boundary sum takes p_x[0], p_y[0], p_z[0] times 0, and
+// the final factor (pb >> 7) is 1, a multiplication that changes no value. The same operations
+// in the same order otherwise, so the same bits (signed zeros included); only the byte decode
+// and the table gather go.
+template <typename C>
+__device__ __forceinline__ C centred_air(C pc, C zp, C zm, C yp, C ym, C xp, C xm, C past,
+                                         const typename CoefOf<C>::T &air, C lam2) {
+#pragma clang fp contract(off)
+  C sb = xm * (C)0;
+  sb = sb + ym * (C)0;
+  sb = sb + zp * (C)0;
+  C s = xm + xp;
+  s = s + ym;
+  s = s + yp;
+  s = s + zp;
+  s = s + zm;
+  s = s + sb;
+  s = s * lam2;
+  C t = s - air.tk * pc;
+  t = t + air.omb * past;
+  return t * air.inv;
+}
+
 // SCHEME 0: forward (Bilbao) boundaries, update types SRL_FORWARD and SHARED; SCHEME 2: the
 // centred (Kowalczyk) boundaries of update type SRL.
 template <typename S, int RY, int TW = kTX, int SCHEME = 0>

@@ -365,6 +365,14 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
     for (int i = 0; i < NB; ++i) m |= ((b[i] & 0xFFu) ^ 0x86u) | (((b[i] >> 16) & 0xFFu) ^ 0x86u);
     return !air_uniform || __ballot(m != 0u) != 0;
   };
+  // the same for the centred scheme's air byte 0x80; its air entry holds for every material
+  // byte, so the material does not matter
+  auto any_noncair = [&](const uint32_t (&b)[NB]) -> bool {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) m |= ((b[i] & 0xFFu) ^ 0x80u) | (((b[i] >> 16) & 0xFFu) ^ 0x80u);
+    return __ballot(m != 0u) != 0;
+  };
   const S *__restrict__ gA = a.A;
   const S *__restrict__ gB = a.B;
   S *__restrict__ gC = a.Cn;
@@ -457,16 +465,29 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
       }
     };
     if constexpr (SCHEME == 2) {
+      // slices whose rows are all air (the air-row bits, or the packed bytes) skip the byte
+      // decode and the table gather of every node
+      auto level1c = [&](auto allair) {
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) {
-        const int r = j + 1;
-        const unsigned pmv = pm_of(bc, j);
-        const C v = Fmt<S>::round(centred_node<C>(pmv & 0xFFu, pmv >> 8, ac[r], ap[r], am[r],
-                                                  ac[r + 1], ac[r - 1], lane_next(ac[r]),
-                                                  lane_prev(ac[r]), past[j], s_coef, ncol, air,
-                                                  lam2));
-        q2[j] = (upd1 && ((ru >> r) & 1u) && col_upd) ? v : (C)0;
-      }
+        for (int j = 0; j < NQ; ++j) {
+          const int r = j + 1;
+          const unsigned pmv = pm_of(bc, j);
+          C w;
+          if constexpr (decltype(allair)::value)
+            w = centred_air<C>(ac[r], ap[r], am[r], ac[r + 1], ac[r - 1], lane_next(ac[r]),
+                               lane_prev(ac[r]), past[j], air, lam2);
+          else
+            w = centred_node<C>(pmv & 0xFFu, pmv >> 8, ac[r], ap[r], am[r], ac[r + 1],
+                                ac[r - 1], lane_next(ac[r]), lane_prev(ac[r]), past[j], s_coef,
+                                ncol, air, lam2);
+          const C v = Fmt<S>::round(w);
+          q2[j] = (upd1 && ((ru >> r) & 1u) && col_upd) ? v : (C)0;
+        }
+      };
+      if (air1 || !any_noncair(bc))
+        level1c(std::true_type());
+      else
+        level1c(std::false_type());
     } else if (air1 ? !air_uniform : any_nonair(bc)) {
       level1(std::true_type());
     } else {
@@ -538,16 +559,27 @@ __global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCH
         }
       };
       if constexpr (SCHEME == 2) {
+        auto level2c = [&](auto allair) {
 #pragma unroll
-        for (int o = 0; o < kFTY; ++o) {
-          const int j = o + 1;
-          const unsigned pmv = pm_of(bp, j);
-          C v = centred_node<C>(pmv & 0xFFu, pmv >> 8, q1[j], q2[j], q0[j], q1[j + 1], q1[j - 1],
-                                lane_next(q1[j]), lane_prev(q1[j]), am[o + 2], s_coef, ncol, air,
-                                lam2);
-          v = (((ru >> (o + 2)) & 1u) && col_upd) ? v : (C)0;
-          if (((rs >> o) & 1u) && owned_lane) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
-        }
+          for (int o = 0; o < kFTY; ++o) {
+            const int j = o + 1;
+            const unsigned pmv = pm_of(bp, j);
+            C v;
+            if constexpr (decltype(allair)::value)
+              v = centred_air<C>(q1[j], q2[j], q0[j], q1[j + 1], q1[j - 1], lane_next(q1[j]),
+                                 lane_prev(q1[j]), am[o + 2], air, lam2);
+            else
+              v = centred_node<C>(pmv & 0xFFu, pmv >> 8, q1[j], q2[j], q0[j], q1[j + 1],
+                                  q1[j - 1], lane_next(q1[j]), lane_prev(q1[j]), am[o + 2],
+                                  s_coef, ncol, air, lam2);
+            v = (((ru >> (o + 2)) & 1u) && col_upd) ? v : (C)0;
+            if (((rs >> o) & 1u) && owned_lane) bst<S>(rD, ro[o + 2], Fmt<S>::raw(v));
+          }
+        };
+        if (any_noncair(bp))
+          level2c(std::false_type());
+        else
+          level2c(std::true_type());
       } else if (any_nonair(bp)) {
         level2(std::true_type());
       } else {
