@@ -275,14 +275,26 @@ __device__ __forceinline__ void srl_tile(
     }
     const C ylo = Fmt<S>::ld(P, base + halo_lo);
     const C yhi = Fmt<S>::ld(P, base + halo_hi);
+    // centred scheme: a wave whose rows are all air (position byte 0x80) skips the decode and
+    // the table gather (centred_air is the same arithmetic)
+    bool wave_air = false;
+    if constexpr (SCHEME == 2) {
+      bool lane_air = true;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) lane_air = lane_air && pb[r] == 0x80u;
+      wave_air = __all(lane_air);
+    }
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const C yp = (r + 1 < RY) ? pc[r + 1] : yhi;
       const C ym = (r > 0) ? pc[r - 1] : ylo;
       C ret;
       if (SCHEME == 2) {
-        ret = centred_node<C>(pb[r], mb[r], pc[r], pn[r], pm[r], yp, ym, xp[r], xm[r], past[r],
-                              coef, cs, air, lam2);
+        if (wave_air)
+          ret = centred_air<C>(pc[r], pn[r], pm[r], yp, ym, xp[r], xm[r], past[r], air, lam2);
+        else
+          ret = centred_node<C>(pb[r], mb[r], pc[r], pn[r], pm[r], yp, ym, xp[r], xm[r], past[r],
+                                coef, cs, air, lam2);
       } else {
         // Sum order z+1, z-1, y+1, y-1, x+1, x-1 (kernels3d.cu:516-525).
         C s = pn[r] + pm[r];
