@@ -1532,11 +1532,12 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   const int64_t bxy =
       (int64_t)((ntx + WX - 1) / WX) * ((Y + kFTY * WY - 1) / (kFTY * WY)) * slots;
   if (zc < 1) {
-    // automatic: the longest of 32 (16 for 2-byte fields), 16 and 8 slices that still gives
-    // 4096 blocks (16 per CU), so small rooms keep every CU busy to the end of the sweep; each
-    // chunk re-reads one slice of P^n on either side, so longer chunks move fewer bytes. On the
-    // 2048x1536x768 room 16-slice chunks are 1 % faster than 32 with 2-byte fields.
-    zc = precision >= 2 ? 16 : 32;
+    // automatic: the longest of 32 (16 for FP64 and 2-byte fields), 16 and 8 slices that still
+    // gives 4096 blocks (16 per CU), so small rooms keep every CU busy to the end of the sweep;
+    // each chunk re-reads one slice of P^n on either side, so longer chunks move fewer bytes. On
+    // the 2048x1536x768 room 16-slice chunks are 1 % faster than 32 with FP64 and 2-byte fields
+    // (and 2 % slower with FP32).
+    zc = precision == PFDTD_F32 ? 32 : 16;
     while (zc > 8 && bxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
   }
   g.zc = zc > zi ? zi : zc;
