@@ -16,6 +16,8 @@ Prints one JSON line:
               `traffic` = measured HBM bytes per launch from the committed rocprofv3 counter
               summary when one exists, and `achieved_measured` / `frac_measured` = those bytes
               over the same launch time (the rate HBM actually moved);
+  shard_share = the same FP32 sweep on one GPU's share of the room under a z-slab
+              decomposition over 8 GPUs (2048 x 1536 x 97: 95 updated slices + 2 halo);
   serving   = p50/p99 latency of the continuous-batching pool on a Poisson stream of
               impulse-response jobs of the C5 kind (rooms 64..512 voxels per side);
   cpu_baseline = the CPU oracle (a C restatement of the reference update loop) timed on this
@@ -159,6 +161,29 @@ def cpu_baseline(scene, steps: int = 300, nz: int = 66) -> dict:
                             "cpu_threads": threads,
                             "cpu_gvox_per_s": round(c1.n_nodes * 2000 / c1_cpu / 1e9, 4),
                             "gpu_gvox_per_s": round(c1.n_nodes * 2000 / c1_gpu / 1e9, 3)}}
+
+
+def shard_share(scene, steps: int, warmup: int, n_gpus: int = 8) -> dict:
+    """One GPU's share of the room under a z-slab decomposition over n_gpus GPUs: Z/n_gpus
+    updated slices plus the two halo slices, here closed by the ceiling material (the update
+    work is the same), FP32, two-level sweep."""
+    Z = scene.labels.shape[0]
+    nz = (Z - 2) // n_gpus + 2
+    lab = np.ascontiguousarray(scene.labels[:nz].copy())
+    lab[nz - 1] = 2
+    pts = rooms.seeded_points(lab, rooms.SplitMix64(97), 9)
+    sc = rooms.Scene(f"{scene.name}_share{n_gpus}", lab, scene.betas, pts[:1],
+                     np.array([1], np.int32), scene.signals[:1], pts[1:], scene.n_steps)
+    with Solver(0) as s:
+        s.load_scene(sc)
+        wall, _ = timed_run(s, steps, warmup, N.F32)
+        kms = kernel_avg_ms(s, min(steps, 100), N.F32)
+    n = lab.size
+    return {"workload": f"z-slab of the room for one of {n_gpus} GPUs: "
+                        f"{lab.shape[2]}x{lab.shape[1]}x{nz} ({nz - 2} updated slices + 2 halo)",
+            "nodes": int(n), "value": round(n * steps / wall / 1e9, 2),
+            "ms_per_step": round(wall * 1e3 / steps, 4), "kernel_ms": round(kms, 4),
+            "roofline_frac": round(n * BYTES_PER_NODE[N.F32] / (kms * 1e-3) / 1e9 / PEAK_GBS, 4)}
 
 
 def bw_probe(bytes_per_array: int, iters: int) -> dict:
@@ -327,6 +352,9 @@ def main() -> int:
                      "bytes_per_node_update": BYTES_PER_NODE[N.F32]},
         "fp16_storage": fp16,
     }
+    if not args.fp32_only and args.scale == 1:
+        out["shard_share"] = shard_share(sc, args.steps, args.warmup)
+        log(f"shard share: {out['shard_share']}")
     if args.serve_jobs > 0:
         out["serving"] = serving_leg(args.serve_jobs, args.serve_rate, args.scale)
         log(f"serving: {out['serving']}")
