@@ -228,3 +228,53 @@ def test_centred_time_levels_bit_exact(gpu, prec, levels):
     assert np.abs(ref).max() > 1e-3
     assert np.array_equal(got, ref), f"max diff {np.abs(got - ref).max()}"
     assert np.array_equal(P, P_ref) and np.array_equal(Pp, Pp_ref)
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+@pytest.mark.parametrize("prec", [N.F32, N.F64, N.F16])
+def test_centred_air_slices_bit_exact(gpu, prec, levels):
+    """A room mostly of plain air (one box), so most wave tiles of most slices take the centred
+    air path (centred_air: no byte decode, no table gather); fields compared bit for bit,
+    signed zeros included, and responses against the oracle."""
+    X, Y, Z = 190, 60, 24
+    lab = rooms.box_room(X, Y, Z, floor=0, ceiling=1, walls=2)
+    lab[6:14, 20:31, 70:95] = 3
+    lab = fill_centred(lab)
+    pos, mat, _ = oracle.classify_kow(lab)
+    # most (slice, tile) windows of the two-level sweep are all centred air (0x80)
+    air = pos == 0x80
+    assert air[2:Z - 2, 2:Y - 2, 2:X - 2].mean() > 0.8
+    rng = np.random.default_rng(7)
+    n_steps = 48
+    src = np.array([[40, 30, 12], [150, 12, 5], [120, 45, 18]], np.int32)
+    kind = np.array([1, 0, 1], np.int32)
+    sig = rng.uniform(-1, 1, (3, n_steps))
+    sig[:, 20:] = 0.0
+    rec = np.array([[45, 33, 12], [100, 20, 9], [180, 50, 20]], np.int32)
+    table = np.zeros((4, 20), np.float32)
+    table[:, 0] = [0.05, 0.2, 0.4, 0.7]
+    pname, storage, scale, _ = PREC[prec]
+    sg = sig if pname == "f64" else sig.astype(np.float32)
+    ref, P_ref, Pp_ref = oracle.run(pos, mat, table, src, kind, sg, rec, n_steps, precision=pname,
+                                    storage=storage, scale_exp=scale,
+                                    literal=oracle.SCHEME_KOWALCZYK, threads=8, want_fields=True)
+    raw = {N.F32: np.float32, N.F64: np.float64, N.F16: np.float16}[prec]
+    bits = {N.F32: np.uint32, N.F64: np.uint64, N.F16: np.uint16}[prec]
+    with Solver() as s:
+        s.set_time_levels(levels, 0)
+        s.set_update_type(N.SRL)
+        s.set_domain_labels(lab)
+        s.set_materials(table)
+        s.set_sources(src, kind, signal_f64=sig, signal_f32=sig.astype(np.float32))
+        s.set_receivers(rec)
+        if scale:
+            s.prepare_fields(prec, scale)
+        s.run(n_steps, prec)
+        got = s.responses(np.float64 if prec == N.F64 else np.float32)
+        P, Pp = s.fields_raw(raw)
+    assert np.abs(ref).max() > 1e-3
+    assert np.array_equal(got, ref), f"max diff {np.abs(got - ref).max()}"
+    if prec == N.F16:
+        P_ref, Pp_ref = P_ref.astype(np.float16), Pp_ref.astype(np.float16)
+    assert np.array_equal(P.view(bits), np.asarray(P_ref, raw).view(bits))
+    assert np.array_equal(Pp.view(bits), np.asarray(Pp_ref, raw).view(bits))
