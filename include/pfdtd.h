@@ -61,7 +61,8 @@ extern "C" {
 #define PFDTD_SWEEP_COLUMN 1 /* blocks of 1 x 4 wave tiles (60 columns x 4 tiles of rows) */
 #define PFDTD_SWEEP_WIDE 2   /* FP32: blocks of 4 x 4 wave tiles (240 columns, 1024 threads) */
 #define PFDTD_SWEEP_PAIR 3   /* 16-bit storage, X even: two columns per lane (124-column wave
-                                tiles, one 32-bit access per pair); otherwise as AUTO */
+                                tiles in blocks of 2 x 2, the difference form 1 x 4; one
+                                32-bit access per pair); otherwise as AUTO */
 
 typedef struct pfdtd_ctx pfdtd_ctx;
 

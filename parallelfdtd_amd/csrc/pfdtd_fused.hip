@@ -698,23 +698,33 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
 // need more than 128 VGPRs and spill. Blocks of 2 or 8 waves and z-chunks of 8 or 32 slices
 // were 1-15 % slower than 4 waves and 16 slices.
 constexpr int kPairTY = 4;
-constexpr int kPairWY = 4;
+// Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
+// sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
+// shapes alternating every 20 launches, three allocation offsets: plain 2x2 2.66 ms, 2x1 2.69,
+// 1x4 2.73, 1x2 2.74, 1x1 2.81, 4x1 2.88, 2x4 2.95; difference form 1x4 and 2x2 2.86 (within
+// 0.3 %), 1x2 2.89, 1x1 2.93. A wave tile past the grid in x (17 tiles of 124 columns on that
+// room) computes on clamped columns and stores nothing; returning early from it, with the
+// block's per-slice barriers kept, measured slower.
+constexpr int kPairWX = 2;
+constexpr int kPairWY = 2;
+constexpr int kDiffWX = 1;
+constexpr int kDiffWY = 4;
 
 // The two-column sweep of plain 16-bit storage (P^n in A, P^{n-1} in B, P^{n+1} to Cn, P^{n+2}
 // to Dn). Per-node selects and lane-masked stores, row offsets in registers, no prefetch: on the
 // 2048x1536x768 room this form is 5 % faster than the branch-free prefetching form of the
 // difference-form kernel below (which issues every buffer access with a scalar row offset, two
 // issue cycles each, to fit its registers).
-template <typename S>
-__global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
+template <typename S, int WX, int WY>
+__global__ __launch_bounds__(64 * WX * WY, 4 / ((WX * WY + 3) / 4)) void srl_update2_pair_kernel(
     Fused2Ctx<S, CoefF, float> p) {
 #pragma clang fp contract(off)
   typedef float C;
-  constexpr int TY = kPairTY, WY = kPairWY;
+  constexpr int TY = kPairTY;
   constexpr int NA = TY + 4;  // P^n rows (2-voxel y halo)
   constexpr int NQ = TY + 2;  // P^{n+1} rows (1-voxel y halo)
-  __shared__ int s_src[WY][kFListCap];
-  __shared__ int s_rec[WY][kFListCap];
+  __shared__ int s_src[WX * WY][kFListCap];
+  __shared__ int s_rec[WX * WY][kFListCap];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   Fused2Args<S, CoefF, float> a = p.a;
@@ -733,19 +743,21 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
     a.Cn += off;
     a.Dn += off;
   }
-  const int xt = (int)(tile % nx);
+  // a block holds WX x WY wave tiles, x fastest; a tile past the grid computes and stores
+  // nothing (its columns clamp to X - 2 and no lane owns a column)
+  const int xt = (int)(tile % nx) * WX + wave % WX;
   const int byi = (int)((tile / nx) % ny);
   const int bzi = (int)(tile / (nx * ny));
   const int xb = xt * kPO;               // first owned column
   const int x0 = xb - 2 + 2 * lane;      // this lane's columns: x0 and x0 + 1
-  const int yt = (byi * WY + wave) * TY;  // first owned row
+  const int yt = (byi * WY + wave / WX) * TY;  // first owned row
   const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);
 
   extern __shared__ __align__(16) unsigned char s_dyn[];
   CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
   const unsigned ncol = (unsigned)a.ncol;
-  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WY)
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WX * WY)
     s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
   const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kPO + 1, yt - 1, yt + TY + 1,
                                 z0 - 1, z1 + 1, s_src[wave], lane, a.src_slot, slot);
@@ -1038,16 +1050,17 @@ __global__ __launch_bounds__(64 * kPairWY, 4) void srl_update2_pair_kernel(
 // row offsets are vector registers (with scalar row offsets every buffer access takes two issue
 // cycles; 1.4 % slower). At three waves per SIMD this is 9 % faster than the selects and
 // lane-masked stores of the plain kernel above.
-__global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
+template <int WX, int WY>
+__global__ __launch_bounds__(64 * WX * WY, 3 / ((WX * WY + 3) / 4)) void srl_update2_pair_diff_kernel(
     Fused2Ctx<h16, CoefF, float> p) {
 #pragma clang fp contract(off)
   typedef h16 S;
   typedef float C;
-  constexpr int TY = kPairTY, WY = kPairWY;
+  constexpr int TY = kPairTY;
   constexpr int NA = TY + 4;  // P^n rows (2-voxel y halo)
   constexpr int NQ = TY + 2;  // P^{n+1} rows (1-voxel y halo)
-  __shared__ int s_src[WY][kFListCap];
-  __shared__ int s_rec[WY][kFListCap];
+  __shared__ int s_src[WX * WY][kFListCap];
+  __shared__ int s_rec[WX * WY][kFListCap];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   Fused2Args<S, CoefF, float> a = p.a;
@@ -1066,12 +1079,14 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
     a.Cn += off;
     a.Dn += off;
   }
-  const int xt = (int)(tile % nx);
+  // a block holds WX x WY wave tiles, x fastest; a tile past the grid computes and stores
+  // nothing (its columns clamp to X - 2 and no lane owns a column)
+  const int xt = (int)(tile % nx) * WX + wave % WX;
   const int byi = (int)((tile / nx) % ny);
   const int bzi = (int)(tile / (nx * ny));
   const int xb = xt * kPO;               // first owned column
   const int x0 = xb - 2 + 2 * lane;      // this lane's columns: x0 and x0 + 1
-  const int yt = (byi * WY + wave) * TY;  // first owned row
+  const int yt = (byi * WY + wave / WX) * TY;  // first owned row
   const int z0 = 1 + bzi * zc;
   const int z1 = min(z0 + zc, Z - 1);
   const bool two = a.levels >= 2;  // the second time level (the odd last step takes one)
@@ -1079,7 +1094,7 @@ __global__ __launch_bounds__(64 * kPairWY, 3) void srl_update2_pair_diff_kernel(
   extern __shared__ __align__(16) unsigned char s_dyn[];
   CoefF *s_coef = reinterpret_cast<CoefF *>(s_dyn);
   const unsigned ncol = (unsigned)a.ncol;
-  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WY)
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * WX * WY)
     s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
   const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 1, xb + kPO + 1, yt - 1, yt + TY + 1,
                                 z0 - 1, z1 + 1, s_src[wave], lane, a.src_slot, slot);
@@ -1515,16 +1530,18 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
            (int64_t)X * Y * 2 < (int64_t)kDropShift &&
            (shape == PFDTD_SWEEP_AUTO || shape == PFDTD_SWEEP_PAIR);
   if (g.pair) {
-    const int rows = kPairTY * kPairWY;
-    const int ptx = (X + kPO - 1) / kPO;
-    const int64_t pxy = (int64_t)ptx * ((Y + rows - 1) / rows) * slots;
+    const int wx = precision == PFDTD_F16_DIFF ? kDiffWX : kPairWX;
+    const int wy = precision == PFDTD_F16_DIFF ? kDiffWY : kPairWY;
+    const int rows = kPairTY * wy;
+    const int pbx = ((X + kPO - 1) / kPO + wx - 1) / wx;
+    const int64_t pxy = (int64_t)pbx * ((Y + rows - 1) / rows) * slots;
     if (zc < 1) {
       zc = 16;
       while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
     }
     g.zc = zc > zi ? zi : zc;
-    g.grid = dim3(ptx, (Y + rows - 1) / rows, ((zi + g.zc - 1) / g.zc) * slots);
-    g.block = dim3(64 * kPairWY, 1, 1);
+    g.grid = dim3(pbx, (Y + rows - 1) / rows, ((zi + g.zc - 1) / g.zc) * slots);
+    g.block = dim3(64 * wx * wy, 1, 1);
     return g;
   }
   const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
@@ -1556,11 +1573,11 @@ static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a
     if (g.pair) {
       if constexpr (std::is_same<S, h16>::value) {
         if (a.diff) {
-          hipLaunchKernelGGL(srl_update2_pair_diff_kernel, g.grid, g.block, lds, stream, p);
+          hipLaunchKernelGGL((srl_update2_pair_diff_kernel<kDiffWX, kDiffWY>), g.grid, g.block, lds, stream, p);
           return hipGetLastError();
         }
       }
-      hipLaunchKernelGGL((srl_update2_pair_kernel<S>), g.grid, g.block, lds, stream, p);
+      hipLaunchKernelGGL((srl_update2_pair_kernel<S, kPairWX, kPairWY>), g.grid, g.block, lds, stream, p);
       return hipGetLastError();
     }
   }
