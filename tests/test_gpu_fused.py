@@ -249,13 +249,14 @@ def pair_border_scene(seed, dims, n_steps=30):
 
 
 @pytest.mark.parametrize("dims", [(250, 40, 18), (372, 27, 14), (126, 25, 12), (8, 9, 7),
-                                  (131, 22, 10)])
+                                  (131, 22, 10), (500, 45, 9)])
 @pytest.mark.parametrize("shape", [N.SWEEP_AUTO, N.SWEEP_PAIR])
 @pytest.mark.parametrize("prec", [N.F16, N.BF16])
 def test_two_level_pair_sweep(gpu, dims, shape, prec):
     """The two-column 16-bit sweep (automatic on rooms of even X) on rooms of two tiles and a
-    pair, exactly three tiles, one tile and a pair, less than one tile, and odd X (where PAIR
-    falls back to one column per lane): bit-exact against the oracle, boundary voxels, sources
+    pair, exactly three tiles, one tile and a pair, less than one tile, odd X (where PAIR falls
+    back to one column per lane) and five tiles (three block columns, the last holding one x
+    tile over the rows of two blocks): bit-exact against the oracle, boundary voxels, sources
     and receivers on the pair and tile borders."""
     sc = pair_border_scene(95, dims)
     got, ref, F, F_ref = run_both(sc, prec, zchunk=5, shape=shape)
