@@ -56,7 +56,8 @@ extern "C" {
 #define PFDTD_SOFT_OVERWRITE 2   /* soft source overwrites, as addSample does (cudaMesh.h:362-366) */
 
 /* ---- block shape of the two-level sweep (pfdtd_set_sweep_shape) -------------------- */
-#define PFDTD_SWEEP_AUTO 0   /* FP32: wide on rooms of at least 960 columns, else column;
+#define PFDTD_SWEEP_AUTO 0   /* on rooms of at least 901 columns (16 tiles of 60): FP32 wide,
+                                FP64 and centred FP32 blocks of 2 x 2 wave tiles; else column;
                                 16-bit storage: pair when X is even, else column */
 #define PFDTD_SWEEP_COLUMN 1 /* blocks of 1 x 4 wave tiles (60 columns x 4 tiles of rows) */
 #define PFDTD_SWEEP_WIDE 2   /* FP32: blocks of 4 x 4 wave tiles (240 columns, 1024 threads) */

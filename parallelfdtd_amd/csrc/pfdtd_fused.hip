@@ -154,20 +154,25 @@ static int fused_ty_host(int precision, int scheme) {
   return scheme == 2 ? fused_ty<h16, 2>() : fused_ty<h16, 0>();
 }
 // A block holds WX x WY wave tiles (x-fastest) that step z together, so that the halo columns
-// and rows they share meet in cache: a column of 1 x 4 tiles, or (WIDE, FP32 only) 4 x 4 tiles,
-// 1024 threads covering 240 columns x 24 rows, one block per CU. On the 2048x1536x768 room the
-// wide FP32 block is 18 % faster than the column (4 x 2: 4 %, 2 x 4: 8 % slower than 4 x 4);
-// on the 512x384x256 room, with 3 of its 12 x tiles past the grid and few blocks per CU, it is
-// 7 % slower. The 16-bit sweep is not bound by HBM traffic: every wider block (2 x 2, 2 x 4,
-// 4 x 2, 4 x 4) was 1-24 % slower than the column. Serving-pool launches (small rooms) use the
+// and rows they share meet in cache: a column of 1 x 4 tiles; wide blocks of 4 x 4 tiles (FP32,
+// forward scheme), 1024 threads covering 240 columns x 24 rows, one block per CU; or square
+// blocks of 2 x 2 tiles (FP64 with the forward scheme, FP32 with the centred one). On the
+// 2048x1536x768 room the wide FP32 block is 18 % faster than the column (4 x 2: 4 %, 2 x 4: 8 %
+// slower than 4 x 4); on the 512x384x256 room, with 3 of its 12 x tiles past the grid and few
+// blocks per CU, it is 7 % slower. Square blocks against the column on the large room, one
+// buffer set, five allocation offsets: FP64 forward 0.8-5.6 % faster, FP32 centred 0.8-6 %
+// faster (2 x 1 blocks: FP64 1.2-2.4 %); FP64 centred within 2 % either way. The 16-bit sweep is
+// not bound by HBM traffic: every wider block (2 x 2, 2 x 4, 4 x 2, 4 x 4) was 1-24 % slower
+// than the column, 6 % with the centred scheme. Serving-pool launches (small rooms) use the
 // column.
-template <bool WIDE>
+template <int SHAPE>
 struct FusedShape {
-  static constexpr int WX = WIDE ? 4 : 1, WY = 4;
+  static constexpr int WX = SHAPE == kWideBlocks ? 4 : SHAPE == kSquareBlocks ? 2 : 1;
+  static constexpr int WY = SHAPE == kSquareBlocks ? 2 : 4;
 };
-template <bool WIDE>
+template <int SHAPE>
 constexpr int fused_waves() {
-  return FusedShape<WIDE>::WX * FusedShape<WIDE>::WY;
+  return FusedShape<SHAPE>::WX * FusedShape<SHAPE>::WY;
 }
 
 // Kernel arguments: one context's domain, or the two-level jobs of one serving-pool launch.
@@ -193,12 +198,12 @@ constexpr int fused_min_waves() {
 // SCHEME 0: forward (Bilbao) boundaries, update types SRL_FORWARD and SHARED; SCHEME 2: the
 // centred (Kowalczyk) boundaries of update type SRL, through the same `centred_node` as the
 // single-step kernel (the reference's operation order).
-template <typename S, typename CT, typename C, bool POOL, bool WIDE, int SCHEME>
-__global__ __launch_bounds__((64 * fused_waves<WIDE>()), (fused_min_waves<S, SCHEME>())) void srl_update2_kernel(
+template <typename S, typename CT, typename C, bool POOL, int SHAPE, int SCHEME>
+__global__ __launch_bounds__((64 * fused_waves<SHAPE>()), (fused_min_waves<S, SCHEME>())) void srl_update2_kernel(
     typename std::conditional<POOL, Fused2Pool, Fused2Ctx<S, CT, C>>::type p) {
 #pragma clang fp contract(off)
   constexpr int kFTY = fused_ty<S, SCHEME>();
-  constexpr int WX = FusedShape<WIDE>::WX, WY = FusedShape<WIDE>::WY, kFWaves = WX * WY;
+  constexpr int WX = FusedShape<SHAPE>::WX, WY = FusedShape<SHAPE>::WY, kFWaves = WX * WY;
   static_assert(kFTY % 2 == 0, "byte planes of two rows share one register");
   constexpr int NA = kFTY + 4;   // P^n rows per wave tile (2-voxel y halo)
   constexpr int NQ = kFTY + 2;   // P^{n+1} rows (1-voxel y halo)
@@ -1520,11 +1525,17 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   g.Z = Z;
   const int zi = Z - 2;
   const int ntx = (X + kFO - 1) / kFO;
-  // automatic: wide blocks for FP32 rooms at least 16 x tiles (960 columns) wide, where at
-  // most 3 of 16 tiles fall past the grid
+  // automatic on rooms at least 16 x tiles (960 columns) wide, where at most 3 of 16 tiles
+  // fall past the grid: wide blocks for FP32 with the forward scheme, square blocks for FP64
+  // with the forward scheme and for FP32 with the centred one
   g.scheme = scheme;
-  g.wide = !pool && precision == 0 && scheme == 0 &&
-           (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16));
+  g.blocks = kColumnBlocks;
+  if (!pool && precision == 0 && scheme == 0 &&
+      (shape == PFDTD_SWEEP_WIDE || (shape == PFDTD_SWEEP_AUTO && ntx >= 16)))
+    g.blocks = kWideBlocks;
+  else if (!pool && shape == PFDTD_SWEEP_AUTO && ntx >= 16 &&
+           ((precision == 1 && scheme == 0) || (precision == 0 && scheme == 2)))
+    g.blocks = kSquareBlocks;
   // 16-bit storage, forward scheme, X even, slices under 2^31 bytes: two columns per lane
   g.pair = !pool && precision >= 2 && scheme == 0 && X % 2 == 0 &&
            (int64_t)X * Y * 2 < (int64_t)kDropShift &&
@@ -1544,8 +1555,12 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
     g.block = dim3(64 * wx * wy, 1, 1);
     return g;
   }
-  const int WX = g.wide ? FusedShape<true>::WX : FusedShape<false>::WX;
-  const int WY = g.wide ? FusedShape<true>::WY : FusedShape<false>::WY;
+  const int WX = g.blocks == kWideBlocks     ? FusedShape<kWideBlocks>::WX
+                 : g.blocks == kSquareBlocks ? FusedShape<kSquareBlocks>::WX
+                                             : FusedShape<kColumnBlocks>::WX;
+  const int WY = g.blocks == kWideBlocks     ? FusedShape<kWideBlocks>::WY
+                 : g.blocks == kSquareBlocks ? FusedShape<kSquareBlocks>::WY
+                                             : FusedShape<kColumnBlocks>::WY;
   const int64_t bxy =
       (int64_t)((ntx + WX - 1) / WX) * ((Y + kFTY * WY - 1) / (kFTY * WY)) * slots;
   if (zc < 1) {
@@ -1582,19 +1597,33 @@ static hipError_t launch_fused(const FusedGeom &g, const Fused2Args<S, CT, C> &a
     }
   }
   if (g.scheme == 2) {
-    hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false, 2>), g.grid, g.block, lds,
-                       stream, p);
+    if constexpr (std::is_same<S, float>::value) {
+      if (g.blocks == kSquareBlocks) {
+        hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, kSquareBlocks, 2>), g.grid,
+                           g.block, lds, stream, p);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, kColumnBlocks, 2>), g.grid, g.block,
+                       lds, stream, p);
     return hipGetLastError();
   }
   if constexpr (std::is_same<S, float>::value) {
-    if (g.wide) {
-      hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, true, 0>), g.grid, g.block, lds,
-                         stream, p);
+    if (g.blocks == kWideBlocks) {
+      hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, kWideBlocks, 0>), g.grid, g.block,
+                         lds, stream, p);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, false, 0>), g.grid, g.block, lds, stream,
-                     p);
+  if constexpr (std::is_same<S, double>::value) {
+    if (g.blocks == kSquareBlocks) {
+      hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, kSquareBlocks, 0>), g.grid,
+                         g.block, lds, stream, p);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((srl_update2_kernel<S, CT, C, false, kColumnBlocks, 0>), g.grid, g.block,
+                     lds, stream, p);
   return hipGetLastError();
 }
 
@@ -1606,16 +1635,16 @@ hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
   const dim3 grid((unsigned)tiles);
   switch (precision) {
   case PFDTD_F32:
-    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true, false, 0>), grid,
-                       dim3(64 * fused_waves<false>()), lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<float, CoefF, float, true, kColumnBlocks, 0>), grid,
+                       dim3(64 * fused_waves<kColumnBlocks>()), lds, stream, p);
     break;
   case PFDTD_F16:
-    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true, false, 0>), grid,
-                       dim3(64 * fused_waves<false>()), lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<h16, CoefF, float, true, kColumnBlocks, 0>), grid,
+                       dim3(64 * fused_waves<kColumnBlocks>()), lds, stream, p);
     break;
   case PFDTD_BF16:
-    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true, false, 0>), grid,
-                       dim3(64 * fused_waves<false>()), lds, stream, p);
+    hipLaunchKernelGGL((srl_update2_kernel<b16, CoefF, float, true, kColumnBlocks, 0>), grid,
+                       dim3(64 * fused_waves<kColumnBlocks>()), lds, stream, p);
     break;
   default:
     return hipErrorInvalidValue;

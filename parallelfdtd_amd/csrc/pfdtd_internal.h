@@ -192,10 +192,12 @@ struct Fused2Args {
   const int32_t *src_slot = nullptr;
   const int32_t *rec_slot = nullptr;
 };
+// Block shapes of the one-column two-level sweep (wave tiles in x by y)
+constexpr int kColumnBlocks = 0, kWideBlocks = 1, kSquareBlocks = 2;
 struct FusedGeom {
   int X, Y, Z;
   int zc;
-  bool wide;   // 4 x 4-wave blocks (FP32, forward scheme), else 1 x 4
+  int blocks;  // kColumnBlocks (1 x 4 wave tiles), kWideBlocks (4 x 4) or kSquareBlocks (2 x 2)
   bool pair;   // 16-bit storage: two columns per lane (124-column wave tiles, X even)
   int scheme;  // 0 forward boundaries, 2 centred
   dim3 grid, block;

@@ -230,6 +230,38 @@ def test_centred_time_levels_bit_exact(gpu, prec, levels):
     assert np.array_equal(P, P_ref) and np.array_equal(Pp, Pp_ref)
 
 
+@pytest.mark.parametrize("prec", [N.F32, N.F64])
+def test_centred_wide_room_bit_exact(gpu, prec):
+    """A room 961 columns wide (17 wave tiles of 60): FP32 takes blocks of 2 x 2 wave tiles there,
+    the last block column with one tile past the grid; FP64 keeps the column of 1 x 4."""
+    sc = centred_border_room(107, dims=(961, 30, 10), n_steps=24)
+    pname, storage, scale, _ = PREC[prec]
+    pos, mat, _ = oracle.classify_kow(sc.labels)
+    sig = sc.signals if pname == "f64" else sc.signals.astype(np.float32)
+    ref, P_ref, Pp_ref = oracle.run(pos, mat, sc.material_table(), sc.src_xyz, sc.src_kind, sig,
+                                    sc.rec_xyz, sc.n_steps, precision=pname, storage=storage,
+                                    scale_exp=scale, literal=oracle.SCHEME_KOWALCZYK, threads=8,
+                                    want_fields=True)
+    raw = np.float64 if prec == N.F64 else np.float32
+    with Solver() as s:
+        s.set_time_levels(2, 0)
+        s.set_update_type(N.SRL)
+        s.set_domain_labels(sc.labels)
+        s.set_materials(sc.material_table())
+        s.set_sources(sc.src_xyz, sc.src_kind, signal_f64=sc.signals,
+                      signal_f32=sc.signals.astype(np.float32))
+        s.set_receivers(sc.rec_xyz)
+        s.run(sc.n_steps, prec)
+        got = s.responses(raw)
+        P, Pp = s.fields_raw(raw)
+    assert np.abs(ref).max() > 1e-3
+    assert np.array_equal(got, ref), f"max diff {np.abs(got - ref).max()}"
+    assert np.array_equal(P.view(np.uint64 if raw == np.float64 else np.uint32),
+                          P_ref.view(np.uint64 if raw == np.float64 else np.uint32))
+    assert np.array_equal(Pp.view(np.uint64 if raw == np.float64 else np.uint32),
+                          Pp_ref.view(np.uint64 if raw == np.float64 else np.uint32))
+
+
 @pytest.mark.parametrize("levels", [1, 2])
 @pytest.mark.parametrize("prec", [N.F32, N.F64, N.F16])
 def test_centred_air_slices_bit_exact(gpu, prec, levels):

@@ -171,6 +171,17 @@ def test_two_level_boundary_voxels_on_tile_borders(gpu, prec):
     assert_same(got, ref, F, F_ref)
 
 
+@pytest.mark.parametrize("prec", [N.F32, N.F64])
+def test_two_level_wide_room_block_shapes(gpu, prec):
+    """A room 961 columns wide (17 wave tiles of 60): FP32 takes blocks of 4 x 4 wave tiles, FP64
+    blocks of 2 x 2, each with tiles past the grid in the last block column; boundary voxels on
+    the tile borders, bit-exact against the oracle."""
+    sc = border_voxel_scene(88, dims=(961, 34, 12), n_steps=24)
+    got, ref, F, F_ref = run_both(sc, prec, zchunk=0)
+    assert np.abs(F_ref[0]).max() > 0
+    assert_same(got, ref, F, F_ref)
+
+
 def test_two_level_domain_reset_with_same_voxel_count(gpu):
     """One context, three domains: an empty room, the same shape with border voxels, and the
     transposed shape (same voxel count): each run equals the oracle, so the packed plane and
