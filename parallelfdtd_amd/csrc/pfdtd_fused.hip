@@ -698,10 +698,10 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
   return (uint32_t)exact_bits<S>(a) | ((uint32_t)exact_bits<S>(b) << 16);
 }
 
-// Rows per wave tile and waves per block (a column of wave tiles) of the two-column sweep.
-// Measured on the 2048x1536x768 room: 4-row tiles (126 VGPRs, four waves per SIMD); 6-row tiles
-// need more than 128 VGPRs and spill. Blocks of 2 or 8 waves and z-chunks of 8 or 32 slices
-// were 1-15 % slower than 4 waves and 16 slices.
+// Rows per wave tile of the two-column sweep. Measured on the 2048x1536x768 room: 4-row tiles
+// (119-126 VGPRs, four waves per SIMD); 6-row tiles need more than 128 VGPRs and spill. In
+// blocks of 2 x 2 waves, z-chunks of 8 / 12 / 24 / 32 slices are 9.7 / 2.4 / 0.0 / 1.3 % slower
+// than 16 and 20 slices 0.4 % faster.
 constexpr int kPairTY = 4;
 // Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
 // sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
@@ -1525,8 +1525,8 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   g.Z = Z;
   const int zi = Z - 2;
   const int ntx = (X + kFO - 1) / kFO;
-  // automatic on rooms at least 16 x tiles (960 columns) wide, where at most 3 of 16 tiles
-  // fall past the grid: wide blocks for FP32 with the forward scheme, square blocks for FP64
+  // automatic on rooms at least 16 x tiles of 60 wide (more than 900 columns), where at most
+  // 3 of 16 tiles fall past the grid: wide blocks for FP32 with the forward scheme, square blocks for FP64
   // with the forward scheme and for FP32 with the centred one
   g.scheme = scheme;
   g.blocks = kColumnBlocks;
