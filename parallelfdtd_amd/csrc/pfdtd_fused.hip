@@ -700,8 +700,8 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
 
 // Rows per wave tile of the two-column sweep. Measured on the 2048x1536x768 room: 4-row tiles
 // (119-126 VGPRs, four waves per SIMD); 6-row tiles need more than 128 VGPRs and spill. In
-// blocks of 2 x 2 waves, z-chunks of 8 / 12 / 24 / 32 slices are 9.7 / 2.4 / 0.0 / 1.3 % slower
-// than 16 and 20 slices 0.4 % faster.
+// blocks of 2 x 2 waves, z-chunks of 8 / 12 / 32 slices are 9.4 / 2.4 / 1.3 % slower than 16,
+// and 20 / 24 slices 0.4 / 0.2 % faster (kept at 16: within the spread of repeated runs).
 constexpr int kPairTY = 4;
 // Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
 // sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
