@@ -285,17 +285,18 @@ int pfdtd_set_kernel_timing(pfdtd_ctx *ctx, int enabled);
  * PROGRESS_MOD in src/kernels/kernels3d.h:36). The reference polls the interrupt every step
  * (kernels3d.cu:86); polling here costs one stream synchronisation per interval. */
 int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
-/* Time levels advanced per sweep of the field (1 or 2; 0 = default = 2, the faster path on
- * MI355X in every precision) and slices per z-chunk of the two-level update (0 = default =
- * automatic: the longest of 32 (16 for 16-bit storage), 16 and 8 slices that gives the sweep
- * 4096 blocks). Batched contexts always advance one level per sweep; both update formulations
- * (forward and centred boundaries) take the two-level sweep. With 2, pfdtd_run advances pairs
- * of steps in one kernel
- * that reads P^n, P^{n-1} and the byte planes once and writes P^{n+1}, P^{n+2} (four field buffers;
- * sources of step n+1 injected and receivers of step n gathered inside the kernel), falling back
- * to single steps for an odd remainder and whenever a source sits on slice z = 0 or z = Z-1.
- * Results are bit-identical to single steps. No reference counterpart: the reference's loop
- * (kernels3d.cu:83-191) launches one update per step. */
+/* Time levels advanced per sweep of the field (1, 2 or 3; 0 = default: 3 for FP32 and FP64
+ * without a batch, else 2 -- the faster settings on MI355X) and slices per z-chunk of the sweep
+ * (0 = automatic: the longest of 32 (16 for 16-bit storage), 16 and 8 slices that gives the
+ * two-level sweep 4096 blocks, the three-level sweep 2048). With 2, pfdtd_run advances pairs of
+ * steps in one kernel that reads P^n, P^{n-1} and the byte planes once and writes P^{n+1} and
+ * P^{n+2}; with 3 (forward update; the centred one and the difference form take 2), triples of
+ * steps in one kernel that writes only P^{n+2} and P^{n+3}. Sources are injected and receivers
+ * gathered inside the kernel; four field buffers. A remainder of one step takes a single-step
+ * update (so does every step while a source sits on slice z = 0 or z = Z-1), four steps left
+ * take two pairs. Non-grouped batches advance one level per sweep. Results are bit-identical
+ * to single steps. No reference counterpart: the reference's loop (kernels3d.cu:83-191)
+ * launches one update per step. */
 int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
 /* Block shape of the two-level sweep: PFDTD_SWEEP_AUTO (default), _COLUMN, _WIDE (FP32 only)
  * or _PAIR (16-bit storage on rooms of even X; elsewhere it acts as AUTO). The wave tiles of a block step z together so their

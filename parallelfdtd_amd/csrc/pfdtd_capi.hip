@@ -76,6 +76,8 @@ struct pfdtd_ctx {
   uint16_t *d_pm = nullptr;  // pos | mat << 8, built for the two-level update (pm_ok: current)
   uint32_t *d_air_rows = nullptr;  // its air-row bits (built with it; null: not available)
   uint32_t *d_air_rows_p = nullptr;  // the same for the two-column sweep's tiles (X even)
+  uint32_t *d_air_rows3 = nullptr;   // the same for the three-level sweep's tiles
+  bool air3_ok = false;              // d_air_rows3 built for the current domain (or no room)
   bool pm_ok = false;
   int64_t dom_cap = 0;  // voxels the position/material buffers hold
   bool has_domain = false;
@@ -330,11 +332,15 @@ int scan_domain(pfdtd_ctx *c) {
   return PFDTD_OK;
 }
 
-// Time levels per sweep: the explicit setting, else two (measured faster than single steps on
-// MI355X for every precision: FP32 7.35 vs 7.74 ms per step on a 2.4 G-voxel room).
+// Time levels per sweep: the explicit setting, else three for FP32 and FP64 without a batch
+// and two otherwise. Same buffers, alternating settings: on the 2048x1536x768 room three levels
+// take FP32 from 4.37 to 3.90 ms per step and FP64 from 8.61 to 7.67, on 1024x768x384 FP32
+// from 0.71 to 0.55 and FP64 from 1.19 to 1.10; 16-bit storage stays on the two-column
+// two-level sweep (2.67 ms against 3.46 for three levels of one column). Two levels were faster
+// than single steps for every precision (FP32 7.35 vs 7.74 ms per step on the large room).
 int levels_for(const pfdtd_ctx *c, int precision) {
-  (void)precision;
   if (c->levels) return c->levels;
+  if ((precision == PFDTD_F32 || precision == PFDTD_F64) && c->lgB == 0) return 3;
   return 2;
 }
 
@@ -362,8 +368,10 @@ int alloc_domain(pfdtd_ctx *c, int64_t X, int64_t Y, int64_t Z) {
     c->dom_cap = 0;
   }
   c->pm_ok = false;
+  c->air3_ok = false;
   dfree(c->d_air_rows);  // sized by the grid's shape, not only its voxel count
   dfree(c->d_air_rows_p);
+  dfree(c->d_air_rows3);
   // fields sized for the previous grid are dropped
   free_fields(c);
   if (!c->d_pos) {
@@ -446,7 +454,7 @@ int ensure_fields(pfdtd_ctx *c, int precision, int scale_exp) {
     return fail(PFDTD_E_ARG, "scale_exp applies to F16/BF16 storage only");
   if (scale_exp < -60 || scale_exp > 60) return fail(PFDTD_E_ARG, "scale_exp out of range");
   const size_t bytes = es * ((size_t)c->N << c->lgB);
-  const bool want4 = (c->lgB == 0 || c->grouped) && levels_for(c, precision) == 2;
+  const bool want4 = (c->lgB == 0 || c->grouped) && levels_for(c, precision) >= 2;
   if (c->d_P && c->field_bytes == bytes && (!want4 || c->d_P2)) {
     c->precision = precision;
     c->scale_exp = scale_exp;
@@ -584,7 +592,7 @@ hipError_t launch_update_any(pfdtd_ctx *c, const UpdateGeom &g) {
 // The two-level update needs the spare pair and no source on the never-updated slices
 // z = 0 and z = Z-1 (their values would have to be carried between the four buffers).
 bool fused_eligible(const pfdtd_ctx *c) {
-  if ((c->lgB > 0 && !c->grouped) || levels_for(c, c->precision) != 2 || !c->d_P2 || !c->d_Pp2)
+  if ((c->lgB > 0 && !c->grouped) || levels_for(c, c->precision) < 2 || !c->d_P2 || !c->d_Pp2)
     return false;
   // the two-level kernel addresses each z-slice through a buffer descriptor (32-bit range)
   const uint64_t esz = c->precision == PFDTD_F64 ? 8 : (c->precision == PFDTD_F32 ? 4 : 2);
@@ -662,6 +670,37 @@ hipError_t launch_update2_any(pfdtd_ctx *c, const FusedGeom &g, int step, int le
   }
 }
 
+// Steps `step`, `step+1` and `step+2` in one sweep (FP32, forward scheme): P^{step+2} lands in
+// d_Pp2 and P^{step+3} in d_P2; the receivers of `step` and `step+1` are gathered and the
+// sources of `step+1` and `step+2` injected inside.
+template <typename S, typename CT, typename C>
+Fused2Args<S, CT, C> fused3_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step) {
+  auto a = fused_args<S>(c, coef, air, lam2, step, false);
+  a.air_rows = c->d_air_rows3;
+  a.levels = 3;
+  a.gather_step2 = c->n_rec > 0 ? step + 1 : -1;
+  a.inject_step2 = (c->n_src > 0 && step + 2 < c->resp_steps) ? step + 2 : -1;
+  return a;
+}
+hipError_t launch_update3(pfdtd_ctx *c, const FusedGeom &g, int step) {
+  switch (c->precision) {
+  case PFDTD_F32:
+    return launch_update3_f32(
+        g, fused3_args<float>(c, c->d_coef_f, c->air_f, c->lam2_f, step), c->stream);
+  case PFDTD_F64:
+    return launch_update3_f64(
+        g, fused3_args<double>(c, c->d_coef_d, c->air_d, c->lam2_d, step), c->stream);
+  case PFDTD_F16:
+    return launch_update3_f16(
+        g, fused3_args<h16>(c, c->d_coef_f, c->air_f, c->lam2_f, step), c->stream);
+  case PFDTD_BF16:
+    return launch_update3_bf16(
+        g, fused3_args<b16>(c, c->d_coef_f, c->air_f, c->lam2_f, step), c->stream);
+  default:
+    return hipErrorInvalidValue;
+  }
+}
+
 int ensure_events(pfdtd_ctx *c, size_t n) {
   while (c->events.size() < n) {
     hipEvent_t e;
@@ -730,6 +769,7 @@ int pfdtd_ctx_destroy(pfdtd_ctx *c) {
   dfree(c->d_pm);
   dfree(c->d_air_rows);
   dfree(c->d_air_rows_p);
+  dfree(c->d_air_rows3);
   dfree(c->d_scan);
   dfree(c->d_coef_f);
   dfree(c->d_coef_d);
@@ -1144,6 +1184,25 @@ static int ensure_pm(pfdtd_ctx *c, bool *fused) {
   return PFDTD_OK;
 }
 
+// The air-row bits of the three-level sweep's tiles, built at its first run on a domain (left
+// null without room for them: the sweep then reads every row's packed value).
+static int ensure_air_rows3(pfdtd_ctx *c) {
+  if (c->air3_ok) return PFDTD_OK;
+  const size_t words = air_rows_words((int)c->X, (int)c->Y, (int)c->Z, 2);
+  if (!c->d_air_rows3 &&
+      hipMalloc((void **)&c->d_air_rows3, sizeof(uint32_t) * words) != hipSuccess) {
+    (void)hipGetLastError();
+    c->d_air_rows3 = nullptr;
+  }
+  if (c->d_air_rows3 && launch_air_rows(c->d_pm, c->d_air_rows3, (int)c->X, (int)c->Y,
+                                        (int)c->Z, c->scheme, 2, c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    dfree(c->d_air_rows3);
+  }
+  c->air3_ok = true;
+  return PFDTD_OK;
+}
+
 int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb interrupt,
               pfdtd_progress_cb progress, void *user, int *steps_done, double *ms_elapsed) {
   int rc = check_ctx(c);
@@ -1190,6 +1249,15 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
                                   diff ? PFDTD_SWEEP_PAIR : c->fshape, c->scheme, batch_slots(c));
   if (diff && (!fused || !fg.pair))
     return fail(PFDTD_E_NOMEM, "no room for the packed plane the two-column sweep reads");
+  // three levels per sweep with the forward scheme when asked for (not the difference form);
+  // the centred scheme takes the two-level sweep
+  const bool three = fused && !diff && c->scheme == 0 && levels_for(c, c->precision) == 3;
+  FusedGeom fg3;
+  if (three) {
+    fg3 = plan_fused3((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, batch_slots(c));
+    rc = ensure_air_rows3(c);
+    if (rc) return rc;
+  }
   int done = 0;
   auto wall0 = std::chrono::steady_clock::now();
   while (done < n_steps) {
@@ -1198,7 +1266,21 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
     int nev = 0;
     for (int k = 0; k < nb;) {
       const int step = done + k;
-      if (fused && k + 2 <= nb) {
+      // four steps left take two two-level sweeps rather than three levels and a single step
+      if (three && k + 3 <= nb && nb - k != 4) {
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev], c->stream));
+        HIP_TRY(launch_update3(c, fg3, step));
+        if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev + 1], c->stream));
+        ++nev;
+        // P^{step+3} (d_P2): gather step+2, inject step+3
+        const int next = step + 3 < n_steps ? step + 3 : -1;
+        HIP_TRY(launch_inject_gather(p_storage(c->precision), c->d_P2,
+                                     inject_args(c, step + 2, next), c->stream));
+        std::swap(c->d_P, c->d_P2);
+        std::swap(c->d_Pp, c->d_Pp2);
+        k += 3;
+        c->upd_launches += 3;
+      } else if (fused && k + 2 <= nb) {
         if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev], c->stream));
         HIP_TRY(launch_update2_any(c, fg, step));
         if (c->kernel_timing) HIP_TRY(hipEventRecord(c->events[2 * nev + 1], c->stream));
@@ -1508,12 +1590,13 @@ int pfdtd_set_update_tiling(pfdtd_ctx *c, int rows, int zchunk) {
 
 int pfdtd_set_time_levels(pfdtd_ctx *c, int levels, int zchunk) {
   if (!c) return fail(PFDTD_E_ARG, "null context");
-  if (levels < 0 || levels > 2) return fail(PFDTD_E_ARG, "time levels %d not in {0,1,2}", levels);
+  if (levels < 0 || levels > 3)
+    return fail(PFDTD_E_ARG, "time levels %d not in {0,1,2,3}", levels);
   if (zchunk < 0 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 0..65535", zchunk);
-  if (levels != c->levels) {
+  if ((levels == 1) != (c->levels == 1)) {
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    free_fields(c);  // reallocated with the right number of buffers at the next run
+    free_fields(c);  // reallocated with the right number of buffers (2 or 4) at the next run
   }
   c->levels = levels;
   c->fzc = zchunk;

@@ -42,6 +42,9 @@ namespace pfdtd {
 constexpr int kFO = 60;       // owned columns per wave tile
 constexpr int kPO = 124;      // owned columns per wave tile of the two-column sweep
 constexpr int kFListCap = 32; // per-wave list capacity for sources / receivers (LDS)
+constexpr int kFO3 = 58;      // owned columns per wave tile of the three-level sweep
+constexpr int kF3TY = 4;      // owned rows per wave tile of the three-level sweep
+constexpr int kThreeBlocks = kSquareBlocks;  // block shape of the three-level sweep
 
 __device__ __forceinline__ bool in_box(const int32_t *q, int x0, int x1, int y0, int y1, int z0,
                                        int z1) {
@@ -625,6 +628,344 @@ __global__ __launch_bounds__((64 * fused_waves<SHAPE>()), (fused_min_waves<S, SC
     body(A1, A2, A0, Q1, Q2, Q0, B1, B2, k);
     if (++k > k1) break;
     body(A2, A0, A1, Q2, Q0, Q1, B2, B0, k);
+    if (++k > k1) break;
+  }
+}
+
+// ---- Three time levels per launch (forward scheme) -------------------------------------------
+// P^{n+1}, P^{n+2} and P^{n+3} in one z-sweep: z-iteration k takes the first level at slice k,
+// the second at k-1 and the third at k-2, from three-slice register queues of P^n, P^{n+1} and
+// P^{n+2}. Only P^{n+2} (Cn) and P^{n+3} (Dn) -- the next launch's past and current fields --
+// are stored, so a launch moves the two-level sweep's bytes for three steps. A wave tile owns
+// 58 columns (lanes 3..60: each level consumes one halo column on either side) and 4 rows (one
+// halo row per level). The sources of steps n+1 and n+2 are injected, and the receivers of
+// steps n and n+1 gathered, inside the sweep in the reference's order; step n+2's gather and
+// step n+3's injection follow on P^{n+3} (launch_inject_gather), as after the two-level sweep.
+// Node arithmetic, sum order and storage rounding are the two-level sweep's, so the results are
+// three single steps bit for bit.
+template <typename S, typename CT, typename C, int SHAPE>
+__global__ __launch_bounds__((64 * fused_waves<SHAPE>()), (fused_min_waves<S, 0>())) void srl_update3_kernel(
+    Fused2Ctx<S, CT, C> p) {
+#pragma clang fp contract(off)
+  constexpr int TY = kF3TY;
+  constexpr int WX = FusedShape<SHAPE>::WX, WY = FusedShape<SHAPE>::WY, kW = WX * WY;
+  constexpr int NA = TY + 6;  // P^n rows yt-3 .. yt+TY+2
+  constexpr int N1 = TY + 4;  // P^{n+1} rows yt-2 .. yt+TY+1
+  constexpr int N2 = TY + 2;  // P^{n+2} rows yt-1 .. yt+TY
+  constexpr int NB = N1 / 2;  // packed byte registers of the N1 rows
+  __shared__ int s_src[kW][kFListCap];
+  __shared__ int s_rec[kW][kFListCap];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  Fused2Args<S, CT, C> a = p.a;
+  const int X = p.X, Y = p.Y, Z = p.Z, zc = p.zc;
+  // XCD-aware tile order and grouped-batch slots, as in srl_update2_kernel
+  const unsigned nx = gridDim.x, ny = gridDim.y, total = nx * ny * gridDim.z;
+  const unsigned lin = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+  const unsigned full = total & ~7u;
+  unsigned tile = lin < full ? (lin & 7u) * (full >> 3) + (lin >> 3) : lin;
+  const int slot = (int)(tile % (unsigned)a.n_slots);
+  tile /= (unsigned)a.n_slots;
+  if (slot) {
+    const int64_t off = (int64_t)slot * a.slot_elems;
+    a.A += off;
+    a.B += off;
+    a.Cn += off;
+    a.Dn += off;
+  }
+  const int bxi = (int)(tile % nx);
+  const int byi = (int)((tile / nx) % ny);
+  const int bzi = (int)(tile / (nx * ny));
+  const int xt = bxi * WX + wave % WX;            // this wave's x tile
+  const int xb = xt * kFO3;                       // first owned column
+  const int x = xb - 3 + lane;                    // this lane's column
+  const int yt = (byi * WY + wave / WX) * TY;     // first owned row
+  const int z0 = 1 + bzi * zc;
+  const int z1 = min(z0 + zc, Z - 1);             // owned slices [z0, z1)
+
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  typedef typename CoefOf<C>::T CE;
+  CE *s_coef = reinterpret_cast<CE *>(s_dyn);
+  const unsigned ncol = (unsigned)a.ncol;
+  for (unsigned i = threadIdx.x; i < kStageRows * ncol; i += 64 * kW)
+    s_coef[i] = a.coef[(i / ncol) * 256u + i % ncol];
+  // sources on this wave's first-level region (owned tile + 2-voxel ring, slices z0-2 .. z1+1);
+  // receivers on its owned tile
+  const int nsrc = wave_collect(a.src_xyz, a.n_src, xb - 2, xb + kFO3 + 2, yt - 2, yt + TY + 2,
+                                z0 - 2, z1 + 2, s_src[wave], lane, a.src_slot, slot);
+  const int nrec = wave_collect(a.rec_xyz, a.n_rec, xb, xb + kFO3, yt, yt + TY, z0, z1,
+                                s_rec[wave], lane, a.rec_slot, slot);
+  __syncthreads();
+
+  const size_t XY = (size_t)X * (size_t)Y;
+  const uint32_t sbytes = (uint32_t)(XY * sizeof(S));  // host guarantees < 2^32
+  const int xc = min(max(x, 0), X - 1);
+  // rows r = 0..NA-1 of the P^n block are y = yt - 3 + r (clamped for loads); row bits as in
+  // srl_update2_kernel
+  uint32_t ro[NA];
+  uint32_t row_upd = 0;
+#pragma unroll
+  for (int r = 0; r < NA; ++r) {
+    const int y = yt - 3 + r;
+    ro[r] = (uint32_t)(min(max(y, 0), Y - 1) * X + xc);
+    if (y > 0 && y < Y - 1) row_upd |= 1u << r;
+  }
+  const bool col_upd = x > 0 && x < X - 1;
+  const bool owned_lane = lane >= 3 && lane < 3 + kFO3 && x < X;
+  uint32_t row_st = 0;
+#pragma unroll
+  for (int o = 0; o < TY; ++o)
+    if (yt + o < Y) row_st |= 1u << o;
+  const double inv_scale = ldexp(1.0, -a.scale_exp);
+  const C scale = (C)ldexp(1.0, a.scale_exp);
+  const CE air = a.air;
+  const C lam2 = a.lam2;
+  auto node = [&](auto table, unsigned pm, C pc, C s, C past) -> C {
+    if constexpr (decltype(table)::value) {
+      const unsigned pb = pm & 0xFFu;
+      const CE e = s_coef[min(pb & 0x7Fu, 6u) * ncol + (pm >> 8)];
+      const C aa = (C)(pb >> 7) * e.inv;
+      return aa * ((e.tk * pc + lam2 * s) - e.omb * past);
+    } else {
+      return air.inv * ((air.tk * pc + lam2 * s) - air.omb * past);
+    }
+  };
+  const bool air_uniform = a.air_uniform != 0;
+  auto any_nonair = [&](const uint32_t (&b)[NB]) -> bool {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) m |= ((b[i] & 0xFFu) ^ 0x86u) | (((b[i] >> 16) & 0xFFu) ^ 0x86u);
+    return !air_uniform || __ballot(m != 0u) != 0;
+  };
+  const S *__restrict__ gA = a.A;
+  const S *__restrict__ gB = a.B;
+  S *__restrict__ gC = a.Cn;
+  S *__restrict__ gD = a.Dn;
+  const uint16_t *__restrict__ gpm = a.pm;
+  // air-row bits of the 58-column tiles (bit b of word w: row 32 w + b - 2), so this wave's
+  // first-level rows yt-2 .. yt+TY+1 start at bit yt
+  typedef const __attribute__((address_space(4))) uint32_t *ConstU32;
+  const uint32_t *__restrict__ gar = a.air_rows;
+  const int nyw = Y / 32 + 2;
+  const size_t ar_slice = (size_t)((X + kFO3 - 1) / kFO3) * (size_t)nyw;
+  const bool past_grid = yt >= Y || xb >= X;
+  const uint32_t *ar_tile = gar && !past_grid ? gar + (size_t)xt * nyw + (yt >> 5) : nullptr;
+  auto air_window = [&](int k) -> uint64_t {
+    if (!gar || k < 1 || k > Z - 2) return 0;
+    if (past_grid) return ~0ull;
+    const ConstU32 w = (ConstU32)(ar_tile + (size_t)k * ar_slice);
+    return (((uint64_t)w[1] << 32) | w[0]) >> (yt & 31);
+  };
+  constexpr uint64_t kAllRows = (1ull << N1) - 1ull;
+  uint64_t ar_win = air_window(z0 - 2);
+
+  // receivers of one level on the values `q` (rows y0 .. y0+n-1) at slice k, then the sources
+  // of step `inj` on them
+  auto gather_inject = [&](auto &q, int y0, int k, int gstep, int inj) {
+    constexpr int n = (int)(sizeof(q) / sizeof(q[0]));
+    if (nrec != 0 && gstep >= 0) {
+      const int cnt = nrec < 0 ? a.n_rec : nrec;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nrec < 0 ? t : s_rec[wave][t];
+        const int32_t *qq = a.rec_xyz + 3 * id;
+        if (qq[2] != k || qq[0] != x) continue;
+        if (nrec < 0 && (!in_box(qq, xb, xb + kFO3, yt, yt + TY, z0, z1) ||
+                         (a.rec_slot && a.rec_slot[id] != slot)))
+          continue;
+#pragma unroll
+        for (int j = 0; j < n; ++j)
+          if (qq[1] == y0 + j) a.resp[(int64_t)id * a.resp_steps + gstep] = (double)q[j] * inv_scale;
+      }
+    }
+    if (nsrc != 0 && inj >= 0) {
+      const int cnt = nsrc < 0 ? a.n_src : nsrc;
+      for (int t = 0; t < cnt; ++t) {
+        const int id = nsrc < 0 ? t : s_src[wave][t];
+        const int32_t *qq = a.src_xyz + 3 * id;
+        if (qq[2] != k || qq[0] != x) continue;
+        if (nsrc < 0 && a.src_slot && a.src_slot[id] != slot) continue;
+        C smp = (C)0;
+        if (inj < a.n_samples) {
+          const int64_t off = (int64_t)id * a.n_samples + inj;
+          if (sizeof(C) == sizeof(double))
+            smp = (C)a.sig_d[off];
+          else
+            smp = (C)a.sig_f[off];
+        }
+        smp = smp * scale;
+        const bool hard = a.src_kind[id] == 0 || a.soft_overwrite;
+#pragma unroll
+        for (int j = 0; j < n; ++j)
+          if (qq[1] == y0 + j) q[j] = hard ? Fmt<S>::round(smp) : Fmt<S>::round(q[j] + smp);
+      }
+    }
+  };
+
+  // am/ac/ap: P^n at k-1, k, k+1; qa/qb/qc: P^{n+1} at k-2, k-1, k; ra/rb/rc: P^{n+2} at
+  // k-3, k-2, k-1; bpp/bp/bc: packed bytes at k-2, k-1, k
+  auto body = [&](C (&am)[NA], C (&ac)[NA], C (&ap)[NA], C (&qa)[N1], C (&qb)[N1], C (&qc)[N1],
+                  C (&ra)[N2], C (&rb)[N2], C (&rc)[N2], uint32_t (&bpp)[NB], uint32_t (&bp)[NB],
+                  uint32_t (&bc)[NB], int k) {
+    uint32_t ru = __builtin_amdgcn_readfirstlane(row_upd);
+    uint32_t rs = __builtin_amdgcn_readfirstlane(row_st);
+    asm volatile("" : "+s"(ru), "+s"(rs));
+    __builtin_amdgcn_s_barrier();
+    const bool upd1 = k >= 1 && k <= Z - 2;
+    if (k + 1 >= 0 && k + 1 <= Z - 1) {
+      const auto rA = slice_rsrc(gA + (size_t)(k + 1) * XY, sbytes);
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ap[r] = Fmt<S>::val(bld<S>(rA, ro[r]));
+    } else {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) ap[r] = (C)0;
+    }
+    C past[N1];
+    const bool air1 = upd1 && (ar_win & kAllRows) == kAllRows;
+    ar_win = air_window(k + 1);
+    if (upd1) {
+      const auto rB = slice_rsrc(gB + (size_t)k * XY, sbytes);
+#pragma unroll
+      for (int j = 0; j < N1; ++j) past[j] = Fmt<S>::val(bld<S>(rB, ro[j + 1]));
+      if (air1) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) bc[i] = 0x00860086u;
+      } else {
+        const auto rPM = slice_rsrc(gpm + (size_t)k * XY, (uint32_t)(XY * 2));
+        uint32_t pmv[N1];
+#pragma unroll
+        for (int j = 0; j < N1; ++j) pmv[j] = bld<uint16_t>(rPM, ro[j + 1]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) bc[i] = pmv[2 * i] | (pmv[2 * i + 1] << 16);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N1; ++j) past[j] = (C)0;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) bc[i] = 0;
+    }
+
+    // ---- first level: P^{n+1}[k], rows yt-2 .. yt+TY+1
+    auto level1 = [&](auto table) {
+#pragma unroll
+      for (int j = 0; j < N1; ++j) {
+        const int r = j + 1;
+        C s = ap[r] + am[r];                 // z+1, z-1
+        s = s + ac[r + 1];                   // y+1
+        s = s + ac[r - 1];                   // y-1
+        s = s + lane_next(ac[r]);            // x+1
+        s = s + lane_prev(ac[r]);            // x-1
+        const C v = Fmt<S>::round(node(table, pm_of(bc, j), ac[r], s, past[j]));
+        qc[j] = (upd1 && ((ru >> r) & 1u) && col_upd) ? v : (C)0;
+      }
+    };
+    if (air1 ? !air_uniform : any_nonair(bc))
+      level1(std::true_type());
+    else
+      level1(std::false_type());
+    gather_inject(qc, yt - 2, k, a.gather_step, a.inject_step);
+
+    // ---- second level: P^{n+2}[k-1], rows yt-1 .. yt+TY
+    const int k2 = k - 1;
+    if (k2 >= z0 - 1) {
+      const bool upd2 = k2 >= 1 && k2 <= Z - 2;
+      auto level2 = [&](auto table) {
+#pragma unroll
+        for (int i = 0; i < N2; ++i) {
+          const int j = i + 1;
+          C s = qc[j] + qa[j];               // z+1, z-1
+          s = s + qb[j + 1];                 // y+1
+          s = s + qb[j - 1];                 // y-1
+          s = s + lane_next(qb[j]);          // x+1
+          s = s + lane_prev(qb[j]);          // x-1
+          const C v = Fmt<S>::round(node(table, pm_of(bp, j), qb[j], s, am[i + 2]));
+          rc[i] = (upd2 && ((ru >> (i + 2)) & 1u) && col_upd) ? v : (C)0;
+        }
+      };
+      if (any_nonair(bp))
+        level2(std::true_type());
+      else
+        level2(std::false_type());
+      gather_inject(rc, yt - 1, k2, a.gather_step2, a.inject_step2);
+      if (k2 >= z0 && k2 < z1) {
+        const auto rC = slice_rsrc(gC + (size_t)k2 * XY, sbytes);
+#pragma unroll
+        for (int o = 0; o < TY; ++o)
+          if (((rs >> o) & 1u) && owned_lane) bst<S>(rC, ro[o + 3], Fmt<S>::raw(rc[o + 1]));
+      }
+    }
+
+    // ---- third level: P^{n+3}[k-2] on the owned rows
+    const int k3 = k - 2;
+    if (k3 >= z0) {
+      const auto rD = slice_rsrc(gD + (size_t)k3 * XY, sbytes);
+      auto level3 = [&](auto table) {
+#pragma unroll
+        for (int o = 0; o < TY; ++o) {
+          const int i = o + 1;
+          C s = rc[i] + ra[i];               // z+1, z-1
+          s = s + rb[i + 1];                 // y+1
+          s = s + rb[i - 1];                 // y-1
+          s = s + lane_next(rb[i]);          // x+1
+          s = s + lane_prev(rb[i]);          // x-1
+          C v = node(table, pm_of(bpp, o + 2), rb[i], s, qa[o + 2]);
+          v = (((ru >> (o + 3)) & 1u) && col_upd) ? v : (C)0;
+          if (((rs >> o) & 1u) && owned_lane) bst<S>(rD, ro[o + 3], Fmt<S>::raw(v));
+        }
+      };
+      if (any_nonair(bpp))
+        level3(std::true_type());
+      else
+        level3(std::false_type());
+    }
+  };
+
+  C A0[NA], A1[NA], A2[NA];
+  C Q0[N1], Q1[N1], Q2[N1];
+  C R0[N2], R1[N2], R2[N2];
+  uint32_t B0[NB], B1[NB], B2[NB];
+  const int k0 = z0 - 2;
+  const int k1 = z1 + 1;
+  if (k0 >= 0) {
+    const auto rK = slice_rsrc(gA + (size_t)k0 * XY, sbytes);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) A1[r] = Fmt<S>::val(bld<S>(rK, ro[r]));
+  } else {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) A1[r] = (C)0;
+  }
+  if (k0 >= 1) {
+    const auto rKm = slice_rsrc(gA + (size_t)(k0 - 1) * XY, sbytes);
+#pragma unroll
+    for (int r = 0; r < NA; ++r) A0[r] = Fmt<S>::val(bld<S>(rKm, ro[r]));
+  } else {
+#pragma unroll
+    for (int r = 0; r < NA; ++r) A0[r] = (C)0;
+  }
+#pragma unroll
+  for (int j = 0; j < N1; ++j) {
+    Q0[j] = (C)0;
+    Q1[j] = (C)0;
+  }
+#pragma unroll
+  for (int i = 0; i < N2; ++i) {
+    R0[i] = (C)0;
+    R1[i] = (C)0;
+    R2[i] = (C)0;
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    B0[i] = 0;
+    B1[i] = 0;
+  }
+  // unrolled by three so the three-slice queues rotate by renaming
+  for (int k = k0;;) {
+    body(A0, A1, A2, Q0, Q1, Q2, R0, R1, R2, B0, B1, B2, k);
+    if (++k > k1) break;
+    body(A1, A2, A0, Q1, Q2, Q0, R1, R2, R0, B1, B2, B0, k);
+    if (++k > k1) break;
+    body(A2, A0, A1, Q2, Q0, Q1, R2, R0, R1, B2, B0, B1, k);
     if (++k > k1) break;
   }
 }
@@ -1475,11 +1816,13 @@ __global__ __launch_bounds__(64 * WX * WY, 3 / ((WX * WY + 3) / 4)) void srl_upd
 // results wave tile t uses -- is plain air (packed value `air`: position 0x86 for the forward
 // scheme, 0x80 for the centred one; material 0); rows outside the grid are set. One wave per
 // word, lanes over the columns; blocks stride over words and slices.
-// The same for the tiles of the two-column sweep (tile width kPO, window [124 t - 1, 124 t + 125)).
+// The same for the tiles of the two-column sweep (tile width kPO, window [124 t - 1, 124 t + 125)),
+// and (h = 2) for those of the three-level sweep: rows y = 32 w + b - 2, columns
+// [58 t - 2, 58 t + 60).
 __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict__ pm,
                                                       uint32_t *__restrict__ out, int X, int Y,
                                                       int Z, int ntx, int nyw, unsigned air,
-                                                      int tw) {
+                                                      int tw, int h) {
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
   for (int z = blockIdx.z; z < Z; z += gridDim.z) {
@@ -1487,10 +1830,10 @@ __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict
     for (int w = blockIdx.y; w < nyw; w += gridDim.y) {
       uint32_t word = 0;
       for (int b = 0; b < 32; ++b) {
-        const int y = 32 * w + b - 1;
+        const int y = 32 * w + b - h;
         bool other = false;
-        for (int c = lane; c < tw + 2; c += 64) {
-          const int x = t * tw - 1 + c;
+        for (int c = lane; c < tw + 2 * h; c += 64) {
+          const int x = t * tw - h + c;
           other = other || (x >= 0 && x < X && y >= 0 && y < Y && slice[(size_t)y * X + x] != air);
         }
         if (__ballot(other) == 0) word |= 1u << b;
@@ -1500,19 +1843,21 @@ __global__ __launch_bounds__(64) void air_rows_kernel(const uint16_t *__restrict
   }
 }
 
-size_t air_rows_words(int X, int Y, int Z, bool pair) {
-  const int tw = pair ? kPO : kFO;
+static int air_rows_tile(int kind) { return kind == 2 ? kFO3 : kind == 1 ? kPO : kFO; }
+
+size_t air_rows_words(int X, int Y, int Z, int kind) {
+  const int tw = air_rows_tile(kind);
   return (size_t)((X + tw - 1) / tw) * (size_t)(Y / 32 + 2) * (size_t)Z;
 }
 
 hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z, int scheme,
-                           bool pair, hipStream_t stream) {
-  const int tw = pair ? kPO : kFO;
+                           int kind, hipStream_t stream) {
+  const int tw = air_rows_tile(kind);
   const int ntx = (X + tw - 1) / tw, nyw = Y / 32 + 2;
   const dim3 grid((unsigned)ntx, (unsigned)std::min(nyw, 4096), (unsigned)std::min(Z, 4096));
   const unsigned air = scheme == 2 ? 0x0080u : 0x0086u;
   hipLaunchKernelGGL(air_rows_kernel, grid, dim3(64), 0, stream, pm, rows, X, Y, Z, ntx, nyw, air,
-                     tw);
+                     tw, kind == 2 ? 2 : 1);
   return hipGetLastError();
 }
 
@@ -1655,6 +2000,78 @@ hipError_t launch_pool_update2(int precision, int tiles, const PoolJobDev *jobs,
 hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
                               hipStream_t stream) {
   return launch_fused(g, a, stream);
+}
+
+FusedGeom plan_fused3(int X, int Y, int Z, int zc, int precision, int slots) {
+  (void)precision;
+  FusedGeom g;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  g.scheme = 0;
+  g.pair = false;
+  g.three = true;
+  g.blocks = kThreeBlocks;
+  const int zi = Z - 2;
+  const int ntx = (X + kFO3 - 1) / kFO3;
+  const int WX = g.blocks == kWideBlocks     ? FusedShape<kWideBlocks>::WX
+                 : g.blocks == kSquareBlocks ? FusedShape<kSquareBlocks>::WX
+                                             : FusedShape<kColumnBlocks>::WX;
+  const int WY = g.blocks == kWideBlocks     ? FusedShape<kWideBlocks>::WY
+                 : g.blocks == kSquareBlocks ? FusedShape<kSquareBlocks>::WY
+                                             : FusedShape<kColumnBlocks>::WY;
+  const int64_t bxy =
+      (int64_t)((ntx + WX - 1) / WX) * ((Y + kF3TY * WY - 1) / (kF3TY * WY)) * slots;
+  if (zc < 1) {
+    // automatic: the longest of 32, 16 and 8 slices that gives 2048 blocks (each chunk also
+    // computes four slices of its neighbours, so shorter chunks cost more than in the two-level
+    // sweep: on 512x384x256 16 slices are 7 % faster than 8, on 130x98x66 8 slices 40 % faster
+    // than 16)
+    zc = 32;
+    while (zc > 8 && bxy * ((zi + zc - 1) / zc) < 2048) zc /= 2;
+  }
+  g.zc = zc > zi ? zi : zc;
+  g.grid = dim3((ntx + WX - 1) / WX, (Y + kF3TY * WY - 1) / (kF3TY * WY),
+                ((zi + g.zc - 1) / g.zc) * slots);
+  g.block = dim3(64 * WX * WY, 1, 1);
+  return g;
+}
+
+template <typename S, typename CT, typename C>
+static hipError_t launch_fused3(const FusedGeom &g, const Fused2Args<S, CT, C> &a,
+                                hipStream_t stream) {
+  const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
+  const Fused2Ctx<S, CT, C> p{a, g.X, g.Y, g.Z, g.zc};
+  switch (g.blocks) {
+  case kWideBlocks:
+    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kWideBlocks>), g.grid, g.block, lds, stream,
+                       p);
+    break;
+  case kSquareBlocks:
+    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kSquareBlocks>), g.grid, g.block, lds,
+                       stream, p);
+    break;
+  default:
+    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kColumnBlocks>), g.grid, g.block, lds,
+                       stream, p);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_update3_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
+                              hipStream_t stream) {
+  return launch_fused3(g, a, stream);
+}
+hipError_t launch_update3_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,
+                              hipStream_t stream) {
+  return launch_fused3(g, a, stream);
+}
+hipError_t launch_update3_f16(const FusedGeom &g, const Fused2Args<h16, CoefF, float> &a,
+                              hipStream_t stream) {
+  return launch_fused3(g, a, stream);
+}
+hipError_t launch_update3_bf16(const FusedGeom &g, const Fused2Args<b16, CoefF, float> &a,
+                               hipStream_t stream) {
+  return launch_fused3(g, a, stream);
 }
 hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,
                               hipStream_t stream) {

@@ -184,6 +184,10 @@ struct Fused2Args {
   int resp_steps;
   int gather_step;
   int levels = 2;  // time levels of this launch (1 only for the difference form's last step)
+  // three-level sweep: the receivers of step gather_step2 are gathered on P^{n+2} before the
+  // sources of step inject_step2 are injected into it (-1: none)
+  int gather_step2 = -1;
+  int inject_step2 = -1;
   int diff = 0;    // 1: the difference form (B holds D = P^n - P^{n-1}; 16-bit storage only)
   // Grouped batch (slot-outermost fields): the launch advances n_slots source configurations,
   // slot s's fields at element offset s * slot_elems; sources and receivers carry their slot.
@@ -199,6 +203,7 @@ struct FusedGeom {
   int zc;
   int blocks;  // kColumnBlocks (1 x 4 wave tiles), kWideBlocks (4 x 4) or kSquareBlocks (2 x 2)
   bool pair;   // 16-bit storage: two columns per lane (124-column wave tiles, X even)
+  bool three = false;  // three time levels per launch (58-column wave tiles, forward scheme)
   int scheme;  // 0 forward boundaries, 2 centred
   dim3 grid, block;
 };
@@ -206,13 +211,16 @@ struct FusedGeom {
 // PFDTD_SWEEP_AUTO / _COLUMN / _WIDE; `scheme`: 0 forward, 2 centred boundaries.
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false,
                      int shape = 0, int scheme = 0, int slots = 1);
+// Grid of the three-level sweep (forward scheme): 58-column, 4-row wave tiles.
+FusedGeom plan_fused3(int X, int Y, int Z, int zc, int precision, int slots = 1);
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.
-// `pair`: the tiles of the two-column sweep (124 columns) instead of 60.
-size_t air_rows_words(int X, int Y, int Z, bool pair = false);
+// `kind`: 0 the one-column two-level sweep's tiles (60 columns), 1 the two-column sweep's (124),
+// 2 the three-level sweep's (58 columns, a 2-voxel window ring).
+size_t air_rows_words(int X, int Y, int Z, int kind = 0);
 hipError_t launch_air_rows(const uint16_t *pm, uint32_t *rows, int X, int Y, int Z, int scheme,
-                           bool pair, hipStream_t stream);
+                           int kind, hipStream_t stream);
 hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
                               hipStream_t stream);
 hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,
@@ -220,6 +228,15 @@ hipError_t launch_update2_f64(const FusedGeom &g, const Fused2Args<double, CoefD
 hipError_t launch_update2_f16(const FusedGeom &g, const Fused2Args<h16, CoefF, float> &a,
                               hipStream_t stream);
 hipError_t launch_update2_bf16(const FusedGeom &g, const Fused2Args<b16, CoefF, float> &a,
+                               hipStream_t stream);
+// Three time levels (the same arguments; P^{n+2} to Cn, P^{n+3} to Dn; forward scheme).
+hipError_t launch_update3_f32(const FusedGeom &g, const Fused2Args<float, CoefF, float> &a,
+                              hipStream_t stream);
+hipError_t launch_update3_f64(const FusedGeom &g, const Fused2Args<double, CoefD, double> &a,
+                              hipStream_t stream);
+hipError_t launch_update3_f16(const FusedGeom &g, const Fused2Args<h16, CoefF, float> &a,
+                              hipStream_t stream);
+hipError_t launch_update3_bf16(const FusedGeom &g, const Fused2Args<b16, CoefF, float> &a,
                                hipStream_t stream);
 
 // Receiver gather of time index `gather_step` (skipped when < 0) followed by injection of

@@ -194,11 +194,12 @@ def centred_border_room(seed, dims=(185, 54, 22), n_steps=36):
     return sc
 
 
-@pytest.mark.parametrize("levels", [1, 2])
+@pytest.mark.parametrize("levels", [1, 2, 3])
 @pytest.mark.parametrize("prec", [N.F32, N.F64, N.F16, N.BF16])
 def test_centred_time_levels_bit_exact(gpu, prec, levels):
-    """The centred update through the two-level sweep (default) and single steps, on a room
-    several wave tiles wide with boundary voxels next to the tile borders."""
+    """The centred update through the two-level sweep (default; also when three levels are
+    asked for) and single steps, on a room several wave tiles wide with boundary voxels next to
+    the tile borders."""
     sc = centred_border_room(105)
     pname, storage, scale, _ = PREC[prec]
     pos, mat, _ = oracle.classify_kow(sc.labels)

@@ -106,16 +106,16 @@ def test_two_level_interrupt(gpu):
 
 
 def test_levels_agree_on_c1(gpu):
-    """Config 1 through both update paths: identical bits."""
+    """Config 1 through single steps, two and three levels per sweep: identical bits."""
     sc = rooms.scene_c1(n_steps=600)
     out = []
-    for levels in (1, 2):
+    for levels in (1, 2, 3):
         with Solver() as s:
             s.set_time_levels(levels)
             s.load_scene(sc)
             s.run(sc.n_steps, N.F32)
             out.append(s.responses(np.float32))
-    assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
 
 
 @pytest.mark.parametrize("prec", [N.F32, N.F16])
@@ -326,3 +326,64 @@ def test_two_level_pair_sweep_other_lambda(gpu, lam):
     assert np.abs(ref).max() > 0
     assert np.array_equal(got, ref)
     assert np.array_equal(P.astype(np.float32), P_ref)
+
+
+def three_level_scene(seed, dims, n_steps):
+    """Single solid voxels next to the 58-column and 4-row borders of the three-level sweep's
+    wave tiles, sources and receivers on the columns either side of those borders (lanes 0-5
+    and 58-63 of a wave) and near the room's last columns and rows."""
+    sc = rooms.scene_random_small(seed, dims=dims, n_mat=3, n_boxes=2, n_src=1, n_rec=1,
+                                  n_steps=n_steps)
+    X, Y, Z = dims
+    xs = sorted({58 * t + d for t in range(1, X // 58 + 2) for d in (-4, -3, -2, -1, 0, 1, 2, 3)}
+                | {2, 3, 4, X - 5, X - 4, X - 3})
+    xs = [x for x in xs if 2 <= x <= X - 3]
+    ys = sorted({m + d for m in range(4, Y, 4) for d in (-2, -1, 0, 1)} & set(range(2, Y - 2)))
+    k = 0
+    for x in xs:
+        for y in ys[::3]:
+            sc.labels[2 + (k * 5) % max(1, Z - 4), y, x] = 2 + k % 2
+            k += 1
+    pts = np.array([[xs[i % len(xs)], ys[(5 * i) % len(ys)], 1 + (3 * i) % (Z - 2)]
+                    for i in range(2 * len(xs))], np.int32)
+    rng = np.random.default_rng(seed)
+    n_src = min(len(pts), 30)
+    sc.src_xyz = pts[:n_src].copy()
+    sc.src_kind = np.array([i % 2 for i in range(n_src)], np.int32)
+    sc.signals = rng.uniform(-1, 1, (n_src, n_steps))
+    sc.rec_xyz = pts[::-1].copy()
+    return sc
+
+
+@pytest.mark.parametrize("dims,n_steps,zchunk", [((180, 41, 19), 31, 5), ((121, 23, 12), 30, 1),
+                                                 ((59, 9, 8), 29, 0), ((7, 6, 5), 16, 2),
+                                                 ((300, 30, 26), 24, 0)])
+def test_three_level_bit_exact(gpu, dims, n_steps, zchunk):
+    """Three time levels per sweep (FP32): bit-exact against the oracle and against two levels,
+    on rooms of several 58-column tiles, a tile and a column, one tile and one column past it,
+    below one tile, and with z-chunks of 1, 2 and 5 slices; step counts that leave 1, 0 and 2
+    steps after the last three-level launch."""
+    sc = three_level_scene(97, dims, n_steps)
+    got, ref, F, F_ref = run_both(sc, N.F32, levels=3, zchunk=zchunk)
+    assert np.abs(ref).max() > 0
+    assert_same(got, ref, F, F_ref)
+    got2, _, F2, _ = run_both(sc, N.F32, levels=2, zchunk=zchunk)
+    assert np.array_equal(got.view(np.uint32), got2.view(np.uint32))
+    assert np.array_equal(F[0].view(np.uint32), F2[0].view(np.uint32))
+
+
+@pytest.mark.parametrize("literal", [1, 2])
+def test_three_level_quirks_and_poll(gpu, literal):
+    """The literal material slot and overwriting soft sources, with progress polls every 7
+    steps (bundles that end between three-level launches)."""
+    sc = three_level_scene(98, (130, 30, 14), 40)
+    assert_same(*run_both(sc, N.F32, levels=3, zchunk=4, literal=literal,
+                          octave=1 if literal & 1 else 0, poll=7))
+
+
+@pytest.mark.parametrize("prec", [N.F64, N.F16, N.BF16])
+def test_three_level_other_precisions(gpu, prec):
+    """Three time levels with FP64 fields and with 16-bit storage (scaled, rounded at every
+    level): bit-exact against the oracle."""
+    sc = three_level_scene(99, (124, 20, 10), 22)
+    assert_same(*run_both(sc, prec, levels=3, zchunk=3))
