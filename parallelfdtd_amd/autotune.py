@@ -2,15 +2,16 @@
 each) in both field layouts -- interleaved source-innermost, advanced by the single-level
 batched update (its tile: y rows per thread, a 256-thread block covering 256/B x-columns of one
 row band, and the z-chunk length), and grouped source-outermost, advanced by the two-level
-sweep in one launch for all slots (block shape and z-chunk) -- timing a short run of each
-candidate on the real domain.
+sweep (block shape and z-chunk) and the three-level sweep (z-chunk) in one launch for all
+slots -- timing a short run of each candidate on the real domain.
 
 Throughput is per source: voxel-updates/s summed over the B slots,
   per_source_gvox = N_nodes * B * steps / seconds / 1e9,
 with the step loop's own HIP-event time (injection and gather included). B = 1 is timed both
 as the unbatched single-level update (the batched kernel's own baseline) and as the default
-unbatched path, the two-time-level sweep (z-chunks `TWO_LEVEL_ZCHUNKS`, column and wide blocks);
-the best of all candidates is the tuner's choice.
+unbatched paths, the two-time-level sweep (z-chunks `TWO_LEVEL_ZCHUNKS`, column and wide blocks)
+and the three-time-level sweep (z-chunks `THREE_LEVEL_ZCHUNKS`, blocks of 2 x 2 wave tiles); the
+best of all candidates is the tuner's choice.
 
 Run:  python -m parallelfdtd_amd.autotune --batches 1,8,16,32,64 --out tune_c4.json
 """
@@ -29,6 +30,7 @@ from .solver import Solver
 
 BYTES_F32 = {"per_voxel": 2, "per_slot": 12}  # 2 + 12*B bytes per voxel-step, FP32 fields
 TWO_LEVEL_ZCHUNKS = (8, 16, 32)
+THREE_LEVEL_ZCHUNKS = (16, 32)
 SHAPES = {"column": N.SWEEP_COLUMN, "wide": N.SWEEP_WIDE}
 
 
@@ -98,6 +100,12 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
                         if cand["per_source_gvox"] > best["per_source_gvox"]:
                             best = cand
                 s.set_sweep_shape(N.SWEEP_AUTO)
+                for zc in THREE_LEVEL_ZCHUNKS:
+                    s.set_time_levels(3, zc)
+                    cand = record(B, 3, None, zc, time_config(s, steps, warmup, N.F32), "square",
+                                  "grouped")
+                    if cand["per_source_gvox"] > best["per_source_gvox"]:
+                        best = cand
                 s.set_time_levels(1)
                 s.set_batch_layout(N.BATCH_INTERLEAVED)
             if B == 1:
@@ -109,6 +117,11 @@ def sweep(scene, batches=(1, 8, 16, 32, 64), rows=(1, 2, 4, 8), zchunks=(4, 8, 1
                         if cand["per_source_gvox"] > best["per_source_gvox"]:
                             best = cand
                 s.set_sweep_shape(N.SWEEP_AUTO)
+                for zc in THREE_LEVEL_ZCHUNKS:
+                    s.set_time_levels(3, zc)
+                    cand = record(B, 3, None, zc, time_config(s, steps, warmup, N.F32), "square")
+                    if cand["per_source_gvox"] > best["per_source_gvox"]:
+                        best = cand
                 s.set_time_levels(1)
             out["best"][str(B)] = best
     for table in (out["best"], out["best_single_level"]):

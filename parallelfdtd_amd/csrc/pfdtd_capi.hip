@@ -332,15 +332,16 @@ int scan_domain(pfdtd_ctx *c) {
   return PFDTD_OK;
 }
 
-// Time levels per sweep: the explicit setting, else three for FP32 and FP64 without a batch
-// and two otherwise. Same buffers, alternating settings: on the 2048x1536x768 room three levels
+// Time levels per sweep: the explicit setting, else three for FP32 and FP64 (unbatched or in
+// the grouped batch layout) and two otherwise. Same buffers, alternating settings: on the 2048x1536x768 room three levels
 // take FP32 from 4.37 to 3.90 ms per step and FP64 from 8.61 to 7.67, on 1024x768x384 FP32
 // from 0.71 to 0.55 and FP64 from 1.19 to 1.10; 16-bit storage stays on the two-column
 // two-level sweep (2.67 ms against 3.46 for three levels of one column). Two levels were faster
 // than single steps for every precision (FP32 7.35 vs 7.74 ms per step on the large room).
 int levels_for(const pfdtd_ctx *c, int precision) {
   if (c->levels) return c->levels;
-  if ((precision == PFDTD_F32 || precision == PFDTD_F64) && c->lgB == 0) return 3;
+  if ((precision == PFDTD_F32 || precision == PFDTD_F64) && (c->lgB == 0 || c->grouped))
+    return 3;
   return 2;
 }
 

@@ -1,7 +1,7 @@
 """The autotuner's bookkeeping on the CPU: a stand-in solver returns fixed step times, and the
 sweep must time single-level candidates for every batch width in the interleaved layout,
-two-level candidates (both block shapes, every z-chunk) unbatched for B = 1 and in the grouped
-layout for B > 1, restore the single-level interleaved setting between them, report the best
+two-level candidates (both block shapes, every z-chunk) and three-level candidates (every
+z-chunk) unbatched for B = 1 and in the grouped layout for B > 1, restore the single-level interleaved setting between them, report the best
 per B with and without the two-level path, and choose the fastest per source."""
 import numpy as np
 import pytest
@@ -93,10 +93,16 @@ def test_sweep_candidates_and_batched_runs_single_level():
         mine = [c for c in two if c["B"] == B]
         assert sorted((c["shape"], c["zchunk"]) for c in mine) == grid
         assert {c["layout"] for c in mine} == {"interleaved" if B == 1 else "grouped"}
-    # interleaved batched runs (B > 1) are timed with one time level, grouped ones with two
+    three = [c for c in rep["candidates"] if c["levels"] == 3]
+    for B in (1, 8, 32):
+        mine = [c for c in three if c["B"] == B]
+        assert sorted(c["zchunk"] for c in mine) == list(autotune.THREE_LEVEL_ZCHUNKS)
+        assert {c["layout"] for c in mine} == {"interleaved" if B == 1 else "grouped"}
+    # interleaved batched runs (B > 1) are timed with one time level, grouped ones with two or
+    # three
     assert all(levels == 1 for (B, lay, levels, _, _) in s.runs
                if B > 1 and lay == N.BATCH_INTERLEAVED)
-    assert all(levels == 2 for (B, lay, levels, _, _) in s.runs if lay == N.BATCH_GROUPED)
+    assert all(levels in (2, 3) for (B, lay, levels, _, _) in s.runs if lay == N.BATCH_GROUPED)
     assert len([c for c in rep["candidates"] if c["levels"] == 1]) == 3 * 2 * 2
 
 

@@ -172,15 +172,16 @@ def test_batch_argument_checks(gpu):
 
 
 def test_autotune_sweep_small_room(gpu):
-    """The tuner times single-level candidates for every B, the two-level path for B = 1 and the
-    grouped two-level layout for B > 1, and chooses the fastest per source."""
+    """The tuner times single-level candidates for every B, the two- and three-level paths for
+    B = 1 and the grouped layout through both for B > 1, and chooses the fastest per source."""
     from parallelfdtd_amd import autotune
     scene = rooms.scene_c4(n_sources=8, n_steps=6, scale=8)
     rep = autotune.sweep(scene, batches=(1, 8), rows=(2,), zchunks=(8,), steps=6, warmup=1,
                          log=lambda m: None)
     levels = sorted((c["B"], c["levels"], c["shape"] or "") for c in rep["candidates"])
-    two = [(2, "column")] * 3 + [(2, "wide")] * 3
-    assert levels == [(1, 1, "")] + [(1,) + t for t in two] + [(8, 1, "")] + [(8,) + t for t in two]
+    multi = [(2, "column")] * 3 + [(2, "wide")] * 3 + [(3, "square")] * 2
+    assert levels == ([(1, 1, "")] + [(1,) + t for t in multi] + [(8, 1, "")]
+                      + [(8,) + t for t in multi])
     assert set(rep["best"]) == {"1", "8"} and set(rep["best_single_level"]) == {"1", "8"}
     assert rep["best_single_level"]["1"]["levels"] == 1
     top = max(rep["candidates"], key=lambda c: c["per_source_gvox"])

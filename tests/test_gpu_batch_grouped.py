@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def run_grouped(sc, B, src_slot, rec_slot, prec, scale_exp=0, zchunk=0, shape=N.SWEEP_AUTO,
-                literal=0, octave=0, fields_dtype=None, steps_plan=None, poll=100):
+                literal=0, octave=0, fields_dtype=None, steps_plan=None, poll=100, levels=2):
     with Solver() as s:
         s.set_domain_labels(sc.labels)
         s.set_materials(sc.material_table(), octave=octave, literal_slot=bool(literal & 1),
@@ -29,7 +29,7 @@ def run_grouped(sc, B, src_slot, rec_slot, prec, scale_exp=0, zchunk=0, shape=N.
         s.set_batch_layout(N.BATCH_GROUPED)
         s.set_source_slots(src_slot)
         s.set_receiver_slots(rec_slot)
-        s.set_time_levels(2, zchunk)
+        s.set_time_levels(levels, zchunk)
         s.set_sweep_shape(shape)
         s.set_poll_interval(poll)
         if prec in (N.F16, N.BF16):
@@ -79,6 +79,23 @@ def test_grouped_slots_match_unbatched_runs(gpu, B, pname, n_steps):
     resp, fields = run_grouped(sc, B, src_slot, rec_slot, PRECS[pname][0], zchunk=5,
                                fields_dtype=dt)
     check_slots(sc, B, src_slot, rec_slot, pname, resp, fields)
+
+
+@pytest.mark.parametrize("n_steps", [60, 62])
+@pytest.mark.parametrize("pname", ["f32", "f64", "f16"])
+def test_grouped_three_levels(gpu, pname, n_steps):
+    """Grouped slots through the three-level sweep (asked for explicitly), with 60 steps (whole
+    three-level launches) and 62 (two steps left: a two-level launch)."""
+    sc = rooms.scene_random_small(151, dims=(130, 25, 15), n_mat=3, n_boxes=4, n_src=6,
+                                  n_rec=9, n_steps=n_steps)
+    B = 4
+    src_slot = np.array([(3 * i) % B for i in range(6)], np.int32)
+    rec_slot = np.array([(i * 5 + 1) % B for i in range(9)], np.int32)
+    scale = 10 if pname == "f16" else 0
+    dt = np.float16 if pname == "f16" else PRECS[pname][2]  # raw 16-bit fields for FP16
+    resp, fields = run_grouped(sc, B, src_slot, rec_slot, PRECS[pname][0], scale_exp=scale,
+                               zchunk=4, fields_dtype=dt, levels=3)
+    check_slots(sc, B, src_slot, rec_slot, pname, resp, fields, scale_exp=scale)
 
 
 @pytest.mark.parametrize("dims", [(130, 30, 14), (131, 30, 14)])
