@@ -224,8 +224,8 @@ int pfdtd_run(pfdtd_ctx *ctx, int precision, int n_steps, pfdtd_interrupt_cb int
 int pfdtd_begin_steps(pfdtd_ctx *ctx, int precision, int n_steps);
 int pfdtd_step(pfdtd_ctx *ctx, int step, int direction);
 /* Steps first_step .. first_step + n_steps - 1 forward in one call: the same fields and
- * responses as that many pfdtd_step(ctx, s, +1) calls, advanced in pairs by the two-level sweep
- * where pfdtd_run would use it (one synchronisation at the end). first_step + n_steps may not
+ * responses as that many pfdtd_step(ctx, s, +1) calls, advanced in triples or pairs by the
+ * three- or two-level sweep where pfdtd_run would use them (one synchronisation at the end). first_step + n_steps may not
  * exceed the response length set by pfdtd_begin_steps. No reference counterpart: the
  * reference's capture loop (App::runCapture, src/App.cpp:350-379) launches one step at a time;
  * FDTD::App::runCapture calls this for the steps between two captures. */

@@ -1412,12 +1412,26 @@ int pfdtd_steps(pfdtd_ctx *c, int first_step, int n_steps) {
   if (rc) return rc;
   const FusedGeom fg = plan_fused((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, false,
                                   c->fshape, c->scheme, batch_slots(c));
+  const bool three = fused && c->scheme == 0 && levels_for(c, c->precision) == 3;
+  FusedGeom fg3;
+  if (three) {
+    fg3 = plan_fused3((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, batch_slots(c));
+    rc = ensure_air_rows3(c);
+    if (rc) return rc;
+  }
   // the sequence of pfdtd_step calls: inject(s), update, gather(s) for s = first .. first+n-1,
   // with the injection of each following step applied right after the previous gather
   HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, first_step), c->stream));
   for (int k = 0; k < n_steps;) {
     const int step = first_step + k;
-    if (fused && k + 2 <= n_steps) {
+    if (three && k + 3 <= n_steps && n_steps - k != 4) {
+      HIP_TRY(launch_update3(c, fg3, step));
+      const int next = k + 3 < n_steps ? step + 3 : -1;
+      HIP_TRY(launch_inject_gather(c->precision, c->d_P2, inject_args(c, step + 2, next), c->stream));
+      std::swap(c->d_P, c->d_P2);
+      std::swap(c->d_Pp, c->d_Pp2);
+      k += 3;
+    } else if (fused && k + 2 <= n_steps) {
       HIP_TRY(launch_update2_any(c, fg, step));
       const int next = k + 2 < n_steps ? step + 2 : -1;
       HIP_TRY(launch_inject_gather(c->precision, c->d_P2, inject_args(c, step + 1, next), c->stream));

@@ -247,9 +247,9 @@ def _step_sequence(sc, prec, plan, scale=0):
 
 @pytest.mark.parametrize("prec,scale", [(N.F32, 0), (N.F16, 9)])
 def test_steps_equal_single_step_calls(gpu, prec, scale):
-    """pfdtd_steps over segments of odd and even lengths (two-level sweep inside) gives the
-    fields and responses of the same steps taken one pfdtd_step call at a time, including after
-    a backward step (the flip rule)."""
+    """pfdtd_steps over segments of odd and even lengths (three- and two-level sweeps inside for
+    FP32, two-level for FP16) gives the fields and responses of the same steps taken one
+    pfdtd_step call at a time, including after a backward step (the flip rule)."""
     sc = rooms.scene_random_small(52, dims=(130, 40, 21), n_mat=3, n_boxes=4, n_src=3, n_rec=5,
                                   n_steps=60)
     one = [("step", n, 1) for n in range(25)] + [("step", 25, -1)] + \
