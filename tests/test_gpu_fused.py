@@ -69,9 +69,10 @@ def test_two_level_quirk_modes(gpu, literal):
     assert_same(*run_both(sc, N.F32, literal=literal, octave=1 if literal & 1 else 0))
 
 
-def test_two_level_crowded_sources_and_receivers(gpu):
+@pytest.mark.parametrize("levels", [2, 3])
+def test_two_level_crowded_sources_and_receivers(gpu, levels):
     """More sources/receivers in one wave tile than its list holds (scan-all path), several
-    sources on one voxel (applied in index order), hard and soft mixed."""
+    sources on one voxel (applied in index order), hard and soft mixed; two and three levels."""
     sc = rooms.scene_random_small(83, dims=(66, 36, 16), n_mat=2, n_boxes=0, n_src=1, n_rec=1,
                                   n_steps=30)
     pts = np.array([[10 + i % 6, 5 + (i // 6) % 5, 4 + i // 30] for i in range(45)], np.int32)
@@ -82,13 +83,14 @@ def test_two_level_crowded_sources_and_receivers(gpu):
     sc.signals = rng.uniform(-1, 1, (len(src), sc.n_steps))
     sc.rec_xyz = pts[::-1].copy()
     for prec in (N.F32, N.F64):
-        assert_same(*run_both(sc, prec, zchunk=4))
+        assert_same(*run_both(sc, prec, levels=levels, zchunk=4))
 
 
-def test_two_level_falls_back_for_boundary_slice_source(gpu):
+@pytest.mark.parametrize("levels", [2, 3])
+def test_two_level_falls_back_for_boundary_slice_source(gpu, levels):
     sc = rooms.scene_random_small(84, dims=(40, 30, 20), n_steps=30)
     sc.src_xyz[0] = [5, 5, 0]  # on the never-updated slice z = 0
-    assert_same(*run_both(sc, N.F32))
+    assert_same(*run_both(sc, N.F32, levels=levels))
 
 
 def test_two_level_interrupt(gpu):
