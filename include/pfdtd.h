@@ -56,10 +56,12 @@ extern "C" {
 #define PFDTD_SOFT_OVERWRITE 2   /* soft source overwrites, as addSample does (cudaMesh.h:362-366) */
 
 /* ---- block shape of the two-level sweep (pfdtd_set_sweep_shape) -------------------- */
-#define PFDTD_SWEEP_AUTO 0   /* on rooms of at least 901 columns (16 tiles of 60): FP32 wide,
-                                FP64 and centred FP32 blocks of 2 x 2 wave tiles; else column;
-                                16-bit storage: pair when X is even, else column */
-#define PFDTD_SWEEP_COLUMN 1 /* blocks of 1 x 4 wave tiles (60 columns x 4 tiles of rows) */
+#define PFDTD_SWEEP_AUTO 0   /* two levels, on rooms of at least 901 columns (16 tiles of 60):
+                                FP32 wide, FP64 and centred FP32 blocks of 2 x 2 wave tiles;
+                                else column; 16-bit storage: pair when X is even, else column;
+                                three levels: blocks of 2 x 2 wave tiles */
+#define PFDTD_SWEEP_COLUMN 1 /* blocks of 1 x 4 wave tiles (60 columns x 4 tiles of rows; the
+                                three-level sweep: 58 columns) */
 #define PFDTD_SWEEP_WIDE 2   /* FP32: blocks of 4 x 4 wave tiles (240 columns, 1024 threads) */
 #define PFDTD_SWEEP_PAIR 3   /* 16-bit storage, X even: two columns per lane (124-column wave
                                 tiles in blocks of 2 x 2, the difference form 1 x 4; one

@@ -1255,7 +1255,8 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   const bool three = fused && !diff && c->scheme == 0 && levels_for(c, c->precision) == 3;
   FusedGeom fg3;
   if (three) {
-    fg3 = plan_fused3((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, batch_slots(c));
+    fg3 = plan_fused3((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, c->fshape,
+                      batch_slots(c));
     rc = ensure_air_rows3(c);
     if (rc) return rc;
   }
@@ -1415,7 +1416,8 @@ int pfdtd_steps(pfdtd_ctx *c, int first_step, int n_steps) {
   const bool three = fused && c->scheme == 0 && levels_for(c, c->precision) == 3;
   FusedGeom fg3;
   if (three) {
-    fg3 = plan_fused3((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, batch_slots(c));
+    fg3 = plan_fused3((int)c->X, (int)c->Y, (int)c->Z, c->fzc, c->precision, c->fshape,
+                      batch_slots(c));
     rc = ensure_air_rows3(c);
     if (rc) return rc;
   }

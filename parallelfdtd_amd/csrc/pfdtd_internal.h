@@ -211,8 +211,9 @@ struct FusedGeom {
 // PFDTD_SWEEP_AUTO / _COLUMN / _WIDE; `scheme`: 0 forward, 2 centred boundaries.
 FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool = false,
                      int shape = 0, int scheme = 0, int slots = 1);
-// Grid of the three-level sweep (forward scheme): 58-column, 4-row wave tiles.
-FusedGeom plan_fused3(int X, int Y, int Z, int zc, int precision, int slots = 1);
+// Grid of the three-level sweep (forward scheme): 58-column, 4-row wave tiles in blocks of
+// 2 x 2 tiles, or (`shape` PFDTD_SWEEP_COLUMN) 1 x 4.
+FusedGeom plan_fused3(int X, int Y, int Z, int zc, int precision, int shape, int slots = 1);
 // One bit per (z-slice, two-level x tile, row): set when the tile's row holds only plain air,
 // so the sweep takes the packed plane's value without loading it. Words for an X x Y x Z grid,
 // and the kernel that builds them from the packed plane.

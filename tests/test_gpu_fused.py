@@ -374,6 +374,12 @@ def test_three_level_bit_exact(gpu, dims, n_steps, zchunk):
     assert np.array_equal(F[0].view(np.uint32), F2[0].view(np.uint32))
 
 
+def test_three_level_column_blocks(gpu):
+    """The three-level sweep in blocks of 1 x 4 wave tiles (PFDTD_SWEEP_COLUMN)."""
+    sc = three_level_scene(96, (180, 41, 19), 27)
+    assert_same(*run_both(sc, N.F32, levels=3, zchunk=6, shape=N.SWEEP_COLUMN))
+
+
 @pytest.mark.parametrize("literal", [1, 2])
 def test_three_level_quirks_and_poll(gpu, literal):
     """The literal material slot and overwriting soft sources, with progress polls every 7
