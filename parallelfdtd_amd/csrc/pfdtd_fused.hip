@@ -2011,7 +2011,6 @@ hipError_t launch_update2_f32(const FusedGeom &g, const Fused2Args<float, CoefF,
 }
 
 FusedGeom plan_fused3(int X, int Y, int Z, int zc, int precision, int shape, int slots) {
-  (void)precision;
   FusedGeom g;
   g.X = X;
   g.Y = Y;
@@ -2029,11 +2028,13 @@ FusedGeom plan_fused3(int X, int Y, int Z, int zc, int precision, int shape, int
   const int64_t bxy =
       (int64_t)((ntx + WX - 1) / WX) * ((Y + kF3TY * WY - 1) / (kF3TY * WY)) * slots;
   if (zc < 1) {
-    // automatic: the longest of 32, 16 and 8 slices that gives 2048 blocks (each chunk also
-    // computes four slices of its neighbours, so shorter chunks cost more than in the two-level
-    // sweep: on 512x384x256 16 slices are 7 % faster than 8, on 130x98x66 8 slices 40 % faster
-    // than 16)
-    zc = 32;
+    // automatic: the longest of 32, 16 and 8 slices (FP64: 96, 48, 24, 12) that gives 2048
+    // blocks (each chunk also computes four slices of its neighbours, so shorter chunks cost
+    // more than in the two-level sweep: on 512x384x256 16 slices are 7 % faster than 8, on
+    // 130x98x66 8 slices 40 % faster than 16). On 2048x1536x768 FP32 takes 3.85 ms per step
+    // with 32 slices (24 / 40 / 48: 3.90 / 3.94 / 3.90); FP64, at two waves per SIMD, 7.37-7.71
+    // with 96 (32 / 48 / 64: 7.83-8.09 / 7.45-7.87 / 7.39-7.79).
+    zc = precision == PFDTD_F64 ? 96 : 32;
     while (zc > 8 && bxy * ((zi + zc - 1) / zc) < 2048) zc /= 2;
   }
   g.zc = zc > zi ? zi : zc;
