@@ -238,6 +238,8 @@ def main() -> int:
     ap.add_argument("--fp32-only", action="store_true", help="only the FP32 run (profiling)")
     ap.add_argument("--fp16-only", action="store_true",
                     help="only the FP16-storage run (profiling); prints no bench line")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="only the FP64 run (three-level sweep; profiling); prints no bench line")
     ap.add_argument("--f16diff-only", action="store_true",
                     help="only the FP16 difference-form run (profiling); prints no bench line")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -283,10 +285,12 @@ def main() -> int:
         s.load_scene(sc)
         log(f"domain classified on GPU in {time.perf_counter() - t0:.1f} s")
 
-        if args.fp16_only or args.f16diff_only:
-            prec, name = (N.F16, "FP16 storage") if args.fp16_only else (N.F16_DIFF,
-                                                                         "FP16 difference form")
-            s.prepare_fields(prec, 12)
+        if args.fp16_only or args.f16diff_only or args.fp64_only:
+            prec, name = ((N.F16, "FP16 storage") if args.fp16_only else
+                          (N.F64, "FP64") if args.fp64_only else
+                          (N.F16_DIFF, "FP16 difference form"))
+            if prec != N.F64:
+                s.prepare_fields(prec, 12)
             wall16, _ = timed_run(s, args.steps, args.warmup, prec)
             k16 = kernel_avg_ms(s, KERNEL_STEPS, prec)
             log(f"{name}: {nodes * args.steps / wall16 / 1e9:.1f} Gvox/s, "
