@@ -192,10 +192,11 @@ class Solver:
         self._c("pfdtd_set_receiver_slots", sl.size, N.ptr(sl))
 
     def set_time_levels(self, levels: int = 0, zchunk: int = 0) -> None:
+        """Time levels per sweep (1, 2 or 3; 0 = automatic) and z-chunk (0 = automatic)."""
         self._c("pfdtd_set_time_levels", levels, zchunk)
 
     def set_sweep_shape(self, shape: int = N.SWEEP_AUTO) -> None:
-        """Block shape of the two-level sweep: N.SWEEP_AUTO, N.SWEEP_COLUMN or N.SWEEP_WIDE."""
+        """Block shape of the sweep: N.SWEEP_AUTO, N.SWEEP_COLUMN, N.SWEEP_WIDE or N.SWEEP_PAIR."""
         self._c("pfdtd_set_sweep_shape", shape)
 
     def set_poll_interval(self, steps: int) -> None:
