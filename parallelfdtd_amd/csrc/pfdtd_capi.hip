@@ -333,11 +333,12 @@ int scan_domain(pfdtd_ctx *c) {
 }
 
 // Time levels per sweep: the explicit setting, else three for FP32 and FP64 (unbatched or in
-// the grouped batch layout) and two otherwise. Same buffers, alternating settings: on the 2048x1536x768 room three levels
-// take FP32 from 4.37 to 3.90 ms per step and FP64 from 8.61 to 7.67, on 1024x768x384 FP32
-// from 0.71 to 0.55 and FP64 from 1.19 to 1.10; 16-bit storage stays on the two-column
-// two-level sweep (2.67 ms against 3.46 for three levels of one column). Two levels were faster
-// than single steps for every precision (FP32 7.35 vs 7.74 ms per step on the large room).
+// the grouped batch layout) and two otherwise. Same buffers, alternating settings: on the
+// 2048x1536x768 room three levels take FP32 from 4.37 to 3.85 ms per step and FP64 from 8.61
+// to 7.4-7.7, on 1024x768x384 FP32 from 0.71 to 0.55 and FP64 from 1.19 to 1.06; 16-bit
+// storage stays on the two-column two-level sweep (2.67 ms against 3.46 for three levels of one
+// column). Two levels were faster than single steps for every precision (FP32 7.35 vs 7.74 ms
+// per step on the large room).
 int levels_for(const pfdtd_ctx *c, int precision) {
   if (c->levels) return c->levels;
   if ((precision == PFDTD_F32 || precision == PFDTD_F64) && (c->lgB == 0 || c->grouped))
@@ -671,8 +672,8 @@ hipError_t launch_update2_any(pfdtd_ctx *c, const FusedGeom &g, int step, int le
   }
 }
 
-// Steps `step`, `step+1` and `step+2` in one sweep (FP32, forward scheme): P^{step+2} lands in
-// d_Pp2 and P^{step+3} in d_P2; the receivers of `step` and `step+1` are gathered and the
+// Steps `step`, `step+1` and `step+2` in one sweep (forward scheme): P^{step+2} lands in d_Pp2
+// and P^{step+3} in d_P2; the receivers of `step` and `step+1` are gathered and the
 // sources of `step+1` and `step+2` injected inside.
 template <typename S, typename CT, typename C>
 Fused2Args<S, CT, C> fused3_args(const pfdtd_ctx *c, const CT *coef, CT air, C lam2, int step) {
