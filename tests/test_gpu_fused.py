@@ -374,6 +374,23 @@ def test_three_level_bit_exact(gpu, dims, n_steps, zchunk):
     assert np.array_equal(F[0].view(np.uint32), F2[0].view(np.uint32))
 
 
+@pytest.mark.parametrize("dims", [(3, 3, 3), (4, 5, 3), (61, 3, 4), (3, 70, 5), (121, 33, 6),
+                                  (59, 9, 4)])
+@pytest.mark.parametrize("prec", [N.F32, N.F64])
+def test_three_level_degenerate_shapes(gpu, dims, prec):
+    """Three levels on rooms down to one interior voxel, one or two interior slices, heights
+    below one wave tile and widths of one tile plus a column."""
+    X, Y, Z = dims
+    sc = rooms.scene_random_small(94, dims=dims, n_mat=2, n_boxes=0, n_src=1, n_rec=1,
+                                  n_steps=13)
+    sc.src_xyz = np.array([[X // 2, Y // 2, Z // 2]], np.int32)
+    sc.src_kind = np.array([0], np.int32)
+    sc.rec_xyz = np.array([[X // 2, Y // 2, Z // 2], [1, 1, 1], [X - 2, Y - 2, Z - 2]], np.int32)
+    got, ref, F, F_ref = run_both(sc, prec, levels=3, zchunk=0)
+    assert np.abs(ref).max() > 0
+    assert_same(got, ref, F, F_ref)
+
+
 def test_three_level_column_blocks(gpu):
     """The three-level sweep in blocks of 1 x 4 wave tiles (PFDTD_SWEEP_COLUMN)."""
     sc = three_level_scene(96, (180, 41, 19), 27)
