@@ -643,7 +643,7 @@ __global__ __launch_bounds__((64 * fused_waves<SHAPE>()), (fused_min_waves<S, SC
 // step n+3's injection follow on P^{n+3} (launch_inject_gather), as after the two-level sweep.
 // Node arithmetic, sum order and storage rounding are the two-level sweep's, so the results are
 // three single steps bit for bit.
-template <typename S, typename CT, typename C, int SHAPE>
+template <typename S, typename CT, typename C, int SHAPE, int BG>
 __global__ __launch_bounds__((64 * fused_waves<SHAPE>()), (fused_min_waves<S, 0>())) void srl_update3_kernel(
     Fused2Ctx<S, CT, C> p) {
 #pragma clang fp contract(off)
@@ -673,15 +673,15 @@ __global__ __launch_bounds__((64 * fused_waves<SHAPE>()), (fused_min_waves<S, 0>
     a.Cn += off;
     a.Dn += off;
   }
-  // x-fastest over bands of two block rows, the two rows of a band interleaved: a block's
-  // y-neighbour in the band is resident with it, while concurrent blocks still stream along x
-  // (DRAM pages). On the 2048x1536x768 room: reads 3.97e10 -> 3.39e10 B per launch, 3.90 ->
-  // 3.85 ms per step; bands of four rows read 3.11e10 but take 3.88 ms, y-fastest order 3.27e10
-  // and 4.94 ms.
+  // x-fastest over bands of BG block rows, the rows of a band interleaved: a block's
+  // y-neighbours in the band are resident with it, while concurrent blocks still stream along x
+  // (DRAM pages). On the 2048x1536x768 room, FP32: reads 3.97e10 -> 3.39e10 B per launch with
+  // bands of two, 3.90 -> 3.85 ms per step (three rows 3.87, four 3.88 at 3.11e10 B, y-fastest
+  // order 4.94 at 3.27e10 B); FP64: 7.66 / 7.34 / 7.26 ms for bands of one / two / three.
   const unsigned tl = tile % (nx * ny);
-  const unsigned band = tl / (2u * nx), in_band = tl % (2u * nx);
-  const unsigned band_rows = min(2u, ny - band * 2u);
-  const int byi = (int)(band * 2u + in_band % band_rows);
+  const unsigned band = tl / ((unsigned)BG * nx), in_band = tl % ((unsigned)BG * nx);
+  const unsigned band_rows = min((unsigned)BG, ny - band * (unsigned)BG);
+  const int byi = (int)(band * (unsigned)BG + in_band % band_rows);
   const int bxi = (int)(in_band / band_rows);
   const int bzi = (int)(tile / (nx * ny));
   const int xt = bxi * WX + wave % WX;            // this wave's x tile
@@ -2050,11 +2050,13 @@ static hipError_t launch_fused3(const FusedGeom &g, const Fused2Args<S, CT, C> &
   const size_t lds = sizeof(CT) * kStageRows * (size_t)a.ncol;
   const Fused2Ctx<S, CT, C> p{a, g.X, g.Y, g.Z, g.zc};
   // square blocks (kThreeBlocks), or the column of 1 x 4 tiles with PFDTD_SWEEP_COLUMN
+  // bands of three block rows for FP64, two otherwise
+  constexpr int BG = sizeof(S) == 8 ? 3 : 2;
   if (g.blocks == kColumnBlocks)
-    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kColumnBlocks>), g.grid, g.block, lds,
+    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kColumnBlocks, BG>), g.grid, g.block, lds,
                        stream, p);
   else
-    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kSquareBlocks>), g.grid, g.block, lds,
+    hipLaunchKernelGGL((srl_update3_kernel<S, CT, C, kSquareBlocks, BG>), g.grid, g.block, lds,
                        stream, p);
   return hipGetLastError();
 }
