@@ -307,9 +307,10 @@ int pfdtd_set_time_levels(pfdtd_ctx *ctx, int levels, int zchunk);
  * three tiles past the grid's last column, so they pay off on wide rooms. Results do not
  * depend on it. No reference counterpart (a tuning knob of the two-level sweep). */
 int pfdtd_set_sweep_shape(pfdtd_ctx *ctx, int shape);
-/* Update-kernel tiling: y rows per thread (1, 2, 4 or 8; default 2) and slices per z-chunk
- * (default 8). Results do not depend on it (every voxel's arithmetic is the same); it only
- * changes the streaming pattern. Replaces the fixed 32x4 block of App::initializeMesh
+/* Update-kernel tiling: y rows per thread (1, 2, 4 or 8) and slices per z-chunk; (0, 0) = the
+ * default, automatic: 4 rows and 4-slice chunks for FP32, 4 and 8 for FP64, 2 and 8 for 16-bit
+ * storage and for interleaved batches. Results do not depend on it (every voxel's arithmetic
+ * is the same); it only changes the streaming pattern. Replaces the fixed 32x4 block of App::initializeMesh
  * (src/App.cpp:193) and the one-block-per-slice grid of launchFDTD3d (kernels3d.cu:42-45). */
 int pfdtd_set_update_tiling(pfdtd_ctx *ctx, int rows, int zchunk);
 int pfdtd_get_kernel_timing(pfdtd_ctx *ctx, double *update_ms_total, int64_t *update_launches);

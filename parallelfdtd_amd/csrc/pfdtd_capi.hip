@@ -101,8 +101,8 @@ struct pfdtd_ctx {
   int update_type = 0;    // enum UpdateType: 0 SRL_FORWARD, 1 SHARED, 2 SRL (centred)
   int scheme = 0;         // boundary formulation: 0 forward (types 0, 1), 2 centred (type 2)
   unsigned degenerate = 0;  // inside voxels without a centred stencil (label classification)
-  int ry = 2;             // y rows per thread of the update kernel
-  int zc = 8;             // slices per z-chunk of the update kernel
+  int ry = 0;             // y rows per thread of the update kernel (0: automatic, plan_single)
+  int zc = 0;             // slices per z-chunk of the update kernel (with ry)
   int levels = 0;         // time levels per sweep: 1, 2, or 0 = by precision
   int fzc = 0;            // slices per z-chunk of the two-level update (0: automatic)
   int fshape = PFDTD_SWEEP_AUTO;  // block shape of the two-level update
@@ -703,6 +703,28 @@ hipError_t launch_update3(pfdtd_ctx *c, const FusedGeom &g, int step) {
   }
 }
 
+// Single-step update tiling: the explicit setting, else by precision for unbatched runs and the
+// grouped layout's single steps (FP32 4 rows per thread x 4-slice chunks, FP64 4 x 8, 16-bit
+// storage 2 x 8) and 2 x 8 for the interleaved batched update. Per single step on the
+// 2048x1536x768 room: FP32 6.97 ms (4 x 4) against 7.19 (2 x 8), FP64 13.38 (4 x 8) against
+// 13.80, FP16 4.72 (2 x 8) against 5.17 (4 x 4), BF16 4.80 against 6.03; interleaved batches of
+// 8-64 slots on 512x384x256 run 13 % slower per source with 4 x 4 than with 2 x 8.
+UpdateGeom plan_single(const pfdtd_ctx *c) {
+  int ry = c->ry, zc = c->zc;
+  const int lgB = c->grouped ? 0 : c->lgB;
+  if (ry == 0) {
+    ry = 2;
+    zc = 8;
+    if (lgB == 0 && c->precision == PFDTD_F32) {
+      ry = 4;
+      zc = 4;
+    } else if (lgB == 0 && c->precision == PFDTD_F64) {
+      ry = 4;
+    }
+  }
+  return plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, ry, zc, lgB);
+}
+
 int ensure_events(pfdtd_ctx *c, size_t n) {
   while (c->events.size() < n) {
     hipEvent_t e;
@@ -1228,8 +1250,7 @@ int pfdtd_run(pfdtd_ctx *c, int precision, int n_steps, pfdtd_interrupt_cb inter
   if (steps_done) *steps_done = 0;
   c->upd_ms = 0.0;
   c->upd_launches = 0;
-  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc,
-                                          c->grouped ? 0 : c->lgB);
+  const UpdateGeom g = plan_single(c);
   const int bundle = c->poll_interval > 0 ? c->poll_interval : kProgressMod;
   if (c->kernel_timing) {
     rc = ensure_events(c, 2 * (size_t)bundle);
@@ -1377,8 +1398,7 @@ int pfdtd_step(pfdtd_ctx *c, int step, int direction) {
   // launchFDTD3dStep: inject this step's samples, update, flip unless the direction changed,
   // then read receivers into response[step] (kernels3d.cu:390-475).
   if (step >= 0) HIP_TRY(launch_inject_gather(c->precision, c->d_P, inject_args(c, -1, step), c->stream));
-  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc,
-                                          c->grouped ? 0 : c->lgB);
+  const UpdateGeom g = plan_single(c);
   HIP_TRY(launch_update_any(c, g));
   if (c->last_direction == direction) std::swap(c->d_P, c->d_Pp);
   c->last_direction = direction;
@@ -1407,8 +1427,7 @@ int pfdtd_steps(pfdtd_ctx *c, int first_step, int n_steps) {
     --n_steps;
     if (n_steps == 0) return PFDTD_OK;
   }
-  const UpdateGeom g = plan_update_batch((int)c->X, (int)c->Y, (int)c->Z, c->ry, c->zc,
-                                          c->grouped ? 0 : c->lgB);
+  const UpdateGeom g = plan_single(c);
   bool fused = fused_eligible(c);
   rc = ensure_pm(c, &fused);
   if (rc) return rc;
@@ -1598,6 +1617,11 @@ int pfdtd_get_kernel_timing(pfdtd_ctx *c, double *update_ms_total, int64_t *upda
 
 int pfdtd_set_update_tiling(pfdtd_ctx *c, int rows, int zchunk) {
   if (!c) return fail(PFDTD_E_ARG, "null context");
+  if (rows == 0 && zchunk == 0) {  // back to the automatic tiling
+    c->ry = 0;
+    c->zc = 0;
+    return PFDTD_OK;
+  }
   if (rows != 1 && rows != 2 && rows != 4 && rows != 8)
     return fail(PFDTD_E_ARG, "rows per thread %d not in {1,2,4,8}", rows);
   if (zchunk < 1 || zchunk > 65535) return fail(PFDTD_E_ARG, "z-chunk %d outside 1..65535", zchunk);

@@ -152,7 +152,8 @@ class Solver:
         self._c("pfdtd_get_kernel_timing", C.byref(ms), C.byref(n))
         return ms.value, n.value
 
-    def set_update_tiling(self, rows: int = 1, zchunk: int = 8) -> None:
+    def set_update_tiling(self, rows: int = 0, zchunk: int = 0) -> None:
+        """Single-step update tiling: rows per thread and z-chunk; (0, 0) = automatic."""
         self._c("pfdtd_set_update_tiling", rows, zchunk)
 
     def transparent_signals(self, signals: np.ndarray, grid_ir: np.ndarray) -> np.ndarray:
