@@ -286,6 +286,14 @@ def test_two_level_automatic_zchunk(gpu, prec):
     with Solver() as s:
         with pytest.raises(N.PfdtdError):
             s.set_time_levels(2, -1)
+        with pytest.raises(N.PfdtdError):
+            s.set_time_levels(4, 0)
+        s.set_time_levels(3, 0)
+        s.set_update_tiling(0, 0)  # back to the automatic single-step tiling
+        with pytest.raises(N.PfdtdError):
+            s.set_update_tiling(0, 8)
+        with pytest.raises(N.PfdtdError):
+            s.set_update_tiling(3, 8)
 
 
 @pytest.mark.parametrize("dims", [(3, 3, 3), (4, 5, 3), (61, 3, 4), (3, 70, 5), (121, 33, 6),
