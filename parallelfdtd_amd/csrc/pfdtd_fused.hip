@@ -1050,7 +1050,7 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
 // Rows per wave tile of the two-column sweep. Measured on the 2048x1536x768 room: 4-row tiles
 // (119-126 VGPRs, four waves per SIMD); 6-row tiles need more than 128 VGPRs and spill. In
 // blocks of 2 x 2 waves, z-chunks of 8 / 12 / 32 slices are 9.4 / 2.4 / 1.3 % slower than 16,
-// and 20 / 24 slices 0.4 / 0.2 % faster (kept at 16: within the spread of repeated runs).
+// and 20 / 24 slices 0.4-0.7 / 0.2-0.3 % faster (20 is the automatic choice).
 constexpr int kPairTY = 4;
 // Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
 // sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
@@ -1900,8 +1900,11 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
     const int pbx = ((X + kPO - 1) / kPO + wx - 1) / wx;
     const int64_t pxy = (int64_t)pbx * ((Y + rows - 1) / rows) * slots;
     if (zc < 1) {
-      zc = 16;
-      while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc /= 2;
+      // 20 slices (the difference form 16), then 16 and 8, while the sweep has fewer than 4096
+      // blocks; on 2048x1536x768 the plain sweep takes 2.641-2.645 ms per step with 20-slice
+      // chunks against 2.659-2.663 with 16 (same buffers, two processes)
+      zc = precision == PFDTD_F16_DIFF ? 16 : 20;
+      while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc = zc == 20 ? 16 : zc / 2;
     }
     g.zc = zc > zi ? zi : zc;
     g.grid = dim3(pbx, (Y + rows - 1) / rows, ((zi + g.zc - 1) / g.zc) * slots);
