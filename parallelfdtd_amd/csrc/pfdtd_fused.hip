@@ -1047,11 +1047,16 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
   return (uint32_t)exact_bits<S>(a) | ((uint32_t)exact_bits<S>(b) << 16);
 }
 
-// Rows per wave tile of the two-column sweep. Measured on the 2048x1536x768 room: 4-row tiles
-// (119-126 VGPRs, four waves per SIMD); 6-row tiles need more than 128 VGPRs and spill. In
-// blocks of 2 x 2 waves, z-chunks of 8 / 12 / 32 slices are 9.4 / 2.4 / 1.3 % slower than 16,
-// and 20 / 24 slices 0.4-0.7 / 0.2-0.3 % faster (20 is the automatic choice).
-constexpr int kPairTY = 4;
+// Rows per wave tile of the two-column sweep: 7 (167 VGPRs, three waves per SIMD). Each level
+// recomputes one halo row on either side and loads two more P^n rows, so taller tiles do less
+// work per owned row for fewer resident waves. Per step on the 2048x1536x768 room, FP16, blocks
+// of 2 x 2 waves, one buffer set: 4 rows (119 VGPRs, four waves) 2.642 ms, 5 rows 2.798, 6 rows
+// 2.622, 7 rows 2.581, 8 rows at two waves 3.388 (at three they spill 72 B per lane); with 7
+// rows 20-slice z-chunks 2.581 and 24 slices 2.590 (4-row tiles: 8 / 12 / 32 slices 9.4 / 2.4 /
+// 1.3 % slower than 16, 20 slices 0.4-0.7 % faster; 20 is the automatic choice).
+constexpr int kPairTY = 7;
+// Rows per wave tile of the difference-form sweep (160 VGPRs, three waves per SIMD).
+constexpr int kDiffTY = 4;
 // Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
 // sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
 // shapes alternating every 20 launches, three allocation offsets: plain 2x2 2.66 ms, 2x1 2.69,
@@ -1070,7 +1075,7 @@ constexpr int kDiffWY = 4;
 // difference-form kernel below (which issues every buffer access with a scalar row offset, two
 // issue cycles each, to fit its registers).
 template <typename S, int WX, int WY>
-__global__ __launch_bounds__(64 * WX * WY, 4 / ((WX * WY + 3) / 4)) void srl_update2_pair_kernel(
+__global__ __launch_bounds__(64 * WX * WY, 3 / ((WX * WY + 3) / 4)) void srl_update2_pair_kernel(
     Fused2Ctx<S, CoefF, float> p) {
 #pragma clang fp contract(off)
   typedef float C;
@@ -1188,7 +1193,8 @@ __global__ __launch_bounds__(64 * WX * WY, 4 / ((WX * WY + 3) / 4)) void srl_upd
   // position|material of both columns in one register per row.
   // The first level at slice k reads A[k+1], B[k] and the packed plane at k: fetch() issues
   // those loads into raw 32-bit registers (pairs of 16-bit values). Issuing slice k+1's loads
-  // one body ahead (20 more VGPRs, three waves per SIMD) was 2 % slower.
+  // one body ahead (4-row tiles: 20 more VGPRs, three waves per SIMD instead of four) was 2 %
+  // slower.
   auto fetch = [&](int k, uint32_t (&ua)[NA], uint32_t (&ub)[NQ], uint32_t (&um)[NQ],
                    bool &uair, uint64_t win) {
     const bool upd = k >= 1 && k <= Z - 2;
@@ -1410,7 +1416,7 @@ __global__ __launch_bounds__(64 * WX * WY, 3 / ((WX * WY + 3) / 4)) void srl_upd
 #pragma clang fp contract(off)
   typedef h16 S;
   typedef float C;
-  constexpr int TY = kPairTY;
+  constexpr int TY = kDiffTY;
   constexpr int NA = TY + 4;  // P^n rows (2-voxel y halo)
   constexpr int NQ = TY + 2;  // P^{n+1} rows (1-voxel y halo)
   __shared__ int s_src[WX * WY][kFListCap];
@@ -1896,7 +1902,7 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
   if (g.pair) {
     const int wx = precision == PFDTD_F16_DIFF ? kDiffWX : kPairWX;
     const int wy = precision == PFDTD_F16_DIFF ? kDiffWY : kPairWY;
-    const int rows = kPairTY * wy;
+    const int rows = (precision == PFDTD_F16_DIFF ? kDiffTY : kPairTY) * wy;
     const int pbx = ((X + kPO - 1) / kPO + wx - 1) / wx;
     const int64_t pxy = (int64_t)pbx * ((Y + rows - 1) / rows) * slots;
     if (zc < 1) {

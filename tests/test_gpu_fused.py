@@ -235,14 +235,14 @@ def test_sweep_shape_rejects_unknown(gpu):
 
 def pair_border_scene(seed, dims, n_steps=30):
     """Single solid voxels next to the 124-column borders of the two-column sweep's wave tiles
-    (and its 4- and 6-row borders), and sources and receivers on both columns of the pairs either
+    (and row borders of 4-, 6- and 7-row tiles), and sources and receivers on both columns of the pairs either
     side of those borders (lanes 0, 1, 62 and 63 of a wave) and on the room's last columns."""
     sc = rooms.scene_random_small(seed, dims=dims, n_mat=3, n_boxes=0, n_src=1, n_rec=1,
                                   n_steps=n_steps)
     X, Y, Z = dims
     borders = {124 * t + d for t in range(1, X // 124 + 2) for d in (-3, -2, -1, 0, 1, 2)}
     xs = sorted((borders | {2, 3, X - 4, X - 3}) & set(range(2, X - 2)))
-    ys = sorted({m + d for ty in (4, 6) for m in range(ty, Y, ty) for d in (-2, -1, 0, 1)}
+    ys = sorted({m + d for ty in (4, 6, 7) for m in range(ty, Y, ty) for d in (-2, -1, 0, 1)}
                 & set(range(2, Y - 2)))
     k = 0
     for x in xs:
