@@ -1059,8 +1059,9 @@ constexpr int kPairTY = 7;
 constexpr int kDiffTY = 4;
 // Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
 // sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
-// shapes alternating every 20 launches, three allocation offsets: plain 2x2 2.66 ms, 2x1 2.69,
-// 1x4 2.73, 1x2 2.74, 1x1 2.81, 4x1 2.88, 2x4 2.95; difference form 1x4 and 2x2 2.86 (within
+// shapes alternating every 20 launches: plain, 7-row tiles, 2x2 2.574 ms, 2x1 2.632, 4x1 2.747,
+// 1x2 2.857, 1x4 2.868, 1x1 2.878 (4-row tiles, three allocation offsets: 2x2 2.66, 2x1 2.69,
+// 1x4 2.73, 1x2 2.74, 1x1 2.81, 4x1 2.88, 2x4 2.95); difference form 1x4 and 2x2 2.86 (within
 // 0.3 %), 1x2 2.89, 1x1 2.93. A wave tile past the grid in x (17 tiles of 124 columns on that
 // room) computes on clamped columns and stores nothing; returning early from it, with the
 // block's per-slice barriers kept, measured slower.
