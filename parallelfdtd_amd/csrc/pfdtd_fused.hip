@@ -1055,7 +1055,8 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
 // rows 20-slice z-chunks 2.581 and 24 slices 2.590 (4-row tiles: 8 / 12 / 32 slices 9.4 / 2.4 /
 // 1.3 % slower than 16, 20 slices 0.4-0.7 % faster; 20 is the automatic choice).
 constexpr int kPairTY = 7;
-// Rows per wave tile of the difference-form sweep (160 VGPRs, three waves per SIMD).
+// Rows per wave tile of the difference-form sweep (160 VGPRs, three waves per SIMD; 5 and 6 rows
+// spill there, and 6 rows at two waves per SIMD take 2.956 against 2.861 ms per step).
 constexpr int kDiffTY = 4;
 // Block shapes (WX x-tiles by WY y-tiles of waves, x fastest): 2 x 2 for the plain two-column
 // sweep, 1 x 4 for the difference form. Per step on the 2048x1536x768 room, one buffer set,
