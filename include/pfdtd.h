@@ -289,7 +289,7 @@ int pfdtd_set_kernel_timing(pfdtd_ctx *ctx, int enabled);
 int pfdtd_set_poll_interval(pfdtd_ctx *ctx, int steps);
 /* Time levels advanced per sweep of the field (1, 2 or 3; 0 = default: 3 for FP32 and FP64
  * (unbatched or grouped batches), else 2 -- the faster settings on MI355X) and slices per z-chunk of the sweep
- * (0 = automatic: the longest of 32 (16-bit storage: 20 for the two-column sweep, 16 for the
+ * (0 = automatic: the longest of 32 (16-bit storage: 22 for the two-column sweep, 16 for the
  * difference form and one column), 16 and 8 slices that gives the two-level sweep 4096 blocks; the three-level sweep 2048 blocks, from 32 slices (FP64: 96)
  * down). With 2, pfdtd_run advances pairs of
  * steps in one kernel that reads P^n, P^{n-1} and the byte planes once and writes P^{n+1} and

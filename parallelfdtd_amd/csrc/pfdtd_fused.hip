@@ -1052,8 +1052,9 @@ __device__ __forceinline__ uint32_t pack_exact(float a, float b) {
 // work per owned row for fewer resident waves. Per step on the 2048x1536x768 room, FP16, blocks
 // of 2 x 2 waves, one buffer set: 4 rows (119 VGPRs, four waves) 2.642 ms, 5 rows 2.798, 6 rows
 // 2.622, 7 rows 2.581, 8 rows at two waves 3.388 (at three they spill 72 B per lane); with 7
-// rows 20-slice z-chunks 2.581 and 24 slices 2.590 (4-row tiles: 8 / 12 / 32 slices 9.4 / 2.4 /
-// 1.3 % slower than 16, 20 slices 0.4-0.7 % faster; 20 is the automatic choice).
+// rows, in another session, 16 / 18 / 20 / 22 / 24-slice z-chunks 2.467 / 2.457 / 2.466 / 2.446
+// / 2.462 (22 is the automatic choice; 4-row tiles: 8 / 12 / 32 slices 9.4 / 2.4 / 1.3 % slower
+// than 16, 20 slices 0.4-0.7 % faster).
 constexpr int kPairTY = 7;
 // Rows per wave tile of the difference-form sweep (160 VGPRs, three waves per SIMD; 5 and 6 rows
 // spill there, and 6 rows at two waves per SIMD take 2.956 against 2.861 ms per step).
@@ -1908,11 +1909,12 @@ FusedGeom plan_fused(int X, int Y, int Z, int zc, int precision, bool pool, int 
     const int pbx = ((X + kPO - 1) / kPO + wx - 1) / wx;
     const int64_t pxy = (int64_t)pbx * ((Y + rows - 1) / rows) * slots;
     if (zc < 1) {
-      // 20 slices (the difference form 16), then 16 and 8, while the sweep has fewer than 4096
-      // blocks; on 2048x1536x768 the plain sweep takes 2.641-2.645 ms per step with 20-slice
-      // chunks against 2.659-2.663 with 16 (same buffers, two processes)
-      zc = precision == PFDTD_F16_DIFF ? 16 : 20;
-      while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc = zc == 20 ? 16 : zc / 2;
+      // 22 slices (the difference form 16), then 16 and 8, while the sweep has fewer than 4096
+      // blocks; on 2048x1536x768 the plain sweep with 7-row tiles takes 2.554-2.555 ms per step
+      // with 22-slice chunks against 2.576-2.578 with 20 (BF16 2.754 against 2.794-2.801; same
+      // buffers, two processes)
+      zc = precision == PFDTD_F16_DIFF ? 16 : 22;
+      while (zc > 8 && pxy * ((zi + zc - 1) / zc) < 4096) zc = zc == 22 ? 16 : zc / 2;
     }
     g.zc = zc > zi ? zi : zc;
     g.grid = dim3(pbx, (Y + rows - 1) / rows, ((zi + g.zc - 1) / g.zc) * slots);
